@@ -1,0 +1,197 @@
+"""Benchmark: TOF list-mode OSEM on one MI355X.
+
+Workload (the headline configuration): scanner radius 424 mm, axial length
+300 mm; image 256x256x128, 2 mm voxels; seeded random-ellipsoid phantom in a
+water body; 5e7 TOF events (400 ps = 60 mm FWHM); 10 subsets; attenuation from
+a mu-map of 0.0096/mm water.  A step is one full OSEM iteration: 10
+sub-iterations, each = TOF projection + fused ratio/backprojection + EM update
+over 5e6 events, so one step processes every event once forward and once back.
+
+value = N_events * steps / t (events/s, fwd+back), t = wall time of the timed
+steps between two device synchronisations.  Setup (event generation,
+attenuation correction, sensitivity image) is outside the timed region.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes per
+launch / mean launch time, HIP events on the launch stream) and a CPU baseline
+(the C oracle on the host cores over a bounded sample of the same events).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TOF OSEM events/s (fwd+back); scatter-est time, surrogate vs SSS at ≤2% err"
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--events", type=float, default=5e7)
+    ap.add_argument("--subsets", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): every box here has one GPU; "
+              "the event-sharded multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from srf_amd import ops, phantom
+    from srf_amd.corrections import attenuation_factors
+    from srf_amd.data import TofConfig, Umap
+    from srf_amd.functions import OsemEngine, uniform_cylinder_lors
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    n = int(args.events)
+    shape, voxel = (256, 256, 128), 2.0
+    radius, axial = 424.0, 300.0
+    fwhm = 60.0
+    tof_bin, tof_sigma2 = 3.66, phantom.tof_sigma2_from_fwhm(fwhm)
+
+    t0 = time.time()
+    act, mu, center, size = phantom.random_ellipsoid_phantom(shape, voxel, args.seed, device=dev)
+    lors = phantom.sample_events(act, center, size, n, radius, axial, args.seed + 1, tof_fwhm_mm=fwhm,
+                                 device=dev)
+    log(f"events generated: {n} in {time.time() - t0:.1f}s")
+    geom = list(center) + list(size)
+    # attenuation correction of the measured values (AttenuationCorrect semantics)
+    y = 1.0 / attenuation_factors(Umap(mu, center, size), lors)
+    # sensitivity image: backprojection of ones over n uniform cylinder LORs
+    emap_k = torch.zeros((shape[2], shape[1], shape[0]), device=dev)
+    n_e, chunk = n, 1 << 23
+    for s in range(0, n_e, chunk):
+        el = uniform_cylinder_lors(min(chunk, n_e - s), radius, axial, 1000 + s, dev)
+        emap_k += ops.backproject(el, None, list(shape), geom, False, 0.0, 0.0, 0)
+    emap_k /= float(n_e)
+    emap = ops.from_kernel_layout(emap_k)
+    torch.cuda.synchronize()
+    log(f"setup done in {time.time() - t0:.1f}s")
+
+    perm = np.random.default_rng(args.seed).permutation(n)
+    eng = OsemEngine(lors, y, emap, shape, geom, args.subsets, perm,
+                     tof=TofConfig(tof_bin, tof_sigma2),
+                     ratio_kind=1, eps=1e-5, zero_guard=1e8)
+    del lors, y
+    # exact term counts T_e per event (same marcher as the kernels; GPU digest)
+    n_sub_ev = eng.n_per_sub
+    l0, _ = eng.subset(0)
+    _, cnt_fp = ops.lor_digest(l0, list(shape), geom, True, tof_bin, tof_sigma2, 1)
+    _, cnt_bp = ops.lor_digest(l0, list(shape), geom, True, tof_bin, tof_sigma2, 0)
+    te_fp = float(cnt_fp.double().mean())
+    te_bp = float(cnt_bp.double().mean())
+    log(f"mean terms per event: fp {te_fp:.1f} bp {te_bp:.1f}")
+
+    # per-kernel timing with HIP events on the launch stream (torch's current stream)
+    stream = torch.cuda.current_stream()
+    ev = []
+
+    def sub_iter(lors_s, y_s, timed):
+        if timed:
+            a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            a.record(stream)
+        p = ops.project(lors_s, eng.x, eng.shape, eng.geom, True, tof_bin, tof_sigma2)
+        if timed:
+            b.record(stream)
+        bp = ops.backproject_ratio(lors_s, y_s, p, 1, 1e-5, eng.shape, eng.geom, True, tof_bin, tof_sigma2, 0)
+        if timed:
+            c.record(stream)
+            ev.append((a, b, c))
+        ops.em_update(eng.x, bp, eng.emap_k, 1e8)
+
+    def step(timed):
+        for k in range(eng.n_sub):
+            l_s, y_s = eng.subset(k)
+            sub_iter(l_s, y_s, timed)
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    log("warmup done")
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(True)
+        log(f"step {i + 1}/{args.steps} issued")
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t_start
+    n_used = eng.n_per_sub * eng.n_sub
+    value = n_used * args.steps / t
+    fp_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
+    bp_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+    nvox = shape[0] * shape[1] * shape[2]
+    # algorithmic bytes: LOR (24) + tof (4) + out/in values + 4 per term, + image once
+    fp_bytes = n_sub_ev * (24 + 4 + 4 + 4 * te_fp) + 4 * nvox
+    bp_bytes = n_sub_ev * (24 + 4 + 8 + 4 * te_bp) + 4 * nvox
+    fp_bw = fp_bytes / (fp_ms * 1e-3)
+    bp_bw = bp_bytes / (bp_ms * 1e-3)
+    dom = ("backproject", bp_ms, bp_bw, bp_bytes) if bp_ms >= fp_ms else ("project", fp_ms, fp_bw, fp_bytes)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_tof_osem_256x256x128.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get(dom[0])
+        except Exception:
+            traffic = None
+    roof = {"bound": "hbm", "achieved": dom[2] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": dom[2] / HBM_PEAK, "traffic": traffic, "kernel": dom[0], "mean_launch_ms": dom[1],
+            "algorithmic_bytes_per_launch": dom[3],
+            "other": {"project_ms": fp_ms, "backproject_ms": bp_ms, "project_GBps": fp_bw / 1e9,
+                      "backproject_GBps": bp_bw / 1e9, "terms_per_event_fp": te_fp, "terms_per_event_bp": te_bp}}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(eng, shape, center, size, tof_bin, tof_sigma2, args.cpu_sample)
+
+    out = {"metric": METRIC, "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": t / args.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "TOF list-mode OSEM, 256x256x128 x 2 mm, 5e7 events, 10 subsets, "
+                                  "R 424 mm, axial 300 mm, TOF FWHM 60 mm (400 ps), water mu-map",
+                      "events": n_used, "subsets": eng.n_sub, "image": list(shape), "voxel_mm": voxel,
+                      "tof_bin_mm": tof_bin, "tof_sigma2_mm2": tof_sigma2, "seed": args.seed},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(eng, shape, center, size, tof_bin, tof_sigma2, sample):
+    """The C oracle (oracle/srf_oracle.c, OpenMP) on the host cores: TOF
+    projection + backprojection of the first `sample` events of subset 0."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    l0, y0 = eng.subset(0)
+    m = min(sample, l0.shape[0])
+    lors = l0[:m].cpu().numpy()
+    img = eng.image().cpu().numpy()
+    log(f"cpu baseline: {m} events, {threads} threads")
+    t = time.perf_counter()
+    p = oracle.project(img, lors, center, size, tof=(tof_bin, tof_sigma2), threads=threads)
+    q = np.where(p + np.float32(1e-5) == 0, 0, y0[:m].cpu().numpy() / (p + np.float32(1e-5))).astype(np.float32)
+    oracle.backproject(q, lors, shape, center, size, tof=(tof_bin, tof_sigma2), f64=False, threads=threads)
+    dt = time.perf_counter() - t
+    return {"value": m / dt, "unit": "events/s", "cores": threads, "kind": "port",
+            "sample": f"first {m} events of subset 0 of the same workload: TOF projection + ratio + "
+                      f"backprojection, {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,136 @@
+/*
+ * srf_hip.h -- C ABI of libsrf_hip.so, the MI355X (gfx950) list-mode PET hot path.
+ *
+ * Conventions shared by every entry point
+ *  - All pointers are device pointers (HBM) unless the comment says otherwise.
+ *    The caller owns every buffer; nothing is allocated by the library except
+ *    where `workspace` is documented as optional.
+ *  - LORs are AoS rows of `lor_stride` floats [x1,y1,z1,x2,y2,z2,(tof,...)] in
+ *    mm, exactly Lors.data of srfnef/data/lors.py:12-22 (stride 6 or 7).
+ *  - TOF values are read from `tof_values[e * tof_stride]`; pass
+ *    `lors + 6` with `tof_stride = 7` to read column 6 in place
+ *    (Lors.tof_values, srfnef/data/lors.py:20-22).
+ *  - Images use the reference kernels' layout: flat index ix + iy*nx + iz*nx*ny
+ *    (srfnef/cc/siddon/siddon.cu.cc:95), i.e. the tf.transpose(image.data) that
+ *    srfnef/functions/project.py:27 hands to the op.
+ *  - `stream` is a hipStream_t (NULL = the legacy default stream).  Every call is
+ *    asynchronous on that stream and host-validated before any launch.
+ *  - Return value: SRF_OK (0) or a negative SRF_ERR_* code; srf_strerror()
+ *    names it.
+ */
+#ifndef SRF_HIP_H
+#define SRF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Image grid: number of voxels, centre and full size in mm.
+ * Replaces the device tensors grid/center/size of the TF ops
+ * (srfnef/cc/siddon/siddon.cc:11-21) -- passed by value, so no D2H copies
+ * (srfnef/cc/siddon/siddon.cu.cc:715-717). */
+typedef struct {
+    int32_t nx, ny, nz;
+    float cx, cy, cz;
+    float sx, sy, sz;
+} srf_grid_t;
+
+/* TOF kernel parameters: attributes tof_bin / tof_sigma2 of ProjectionTof
+ * (srfnef/cc/siddon_tof/siddon_tof.cc:18-19), TofConfig defaults 3.66 / 1000
+ * (srfnef/geometry/tof_config.py:16-17).  NULL selects the non-TOF kernels. */
+typedef struct {
+    float tof_bin;
+    float tof_sigma2;
+} srf_tof_t;
+
+/* Backprojection weight set. */
+typedef enum {
+    /* Bit-compatible with Backprojection / BackprojectionTof: weight without
+     * 1/L^2, dominant test '>', no segment cuts (siddon.cu.cc:342-656,
+     * siddon_tof.cu.cc:371-707). */
+    SRF_BP_REFERENCE = 0,
+    /* Exact transpose of srf_project: the projector's predicates, cuts and 1/L^2
+     * weight, scattered instead of gathered.  Backward of the autograd projector. */
+    SRF_BP_ADJOINT = 1
+} srf_bp_mode_t;
+
+/* Ratio kinds of the EM drivers (reference file:line per kind). */
+typedef enum {
+    SRF_RATIO_PLAIN = 0,      /* q = y / (p + eps): mlem.py:41-42 (eps = mean(y)*1e-8), osem_full.py:219-220 (1e-8) */
+    SRF_RATIO_DIV_NO_NAN = 1  /* q = (p+eps)==0 ? 0 : y/(p+eps): mlem_full.py:68 (eps 0), mlem_full_tof.py:109 (1e-5) */
+} srf_ratio_kind_t;
+
+#define SRF_OK 0
+#define SRF_ERR_INVALID_ARG (-1)   /* null pointer, negative size, bad stride or grid */
+#define SRF_ERR_UNSUPPORTED (-2)   /* e.g. image larger than the 2^31 voxel index space */
+#define SRF_ERR_LAUNCH (-3)        /* a HIP launch or API call failed */
+#define SRF_ERR_WORKSPACE (-4)     /* workspace too small */
+
+const char *srf_strerror(int code);
+int srf_version(void);
+
+/* Forward projection (Projection / ProjectionTof, siddon.cc:11-21,
+ * siddon_tof.cc:11-24; host fn projection(), siddon.cu.cc:706-728 and
+ * siddon_tof.cu.cc:761-785).  out[e] = sum_v image[v] * w_ev for every event;
+ * events the reference skips get 0.  `out` is fully overwritten. */
+int srf_project(const float *lors, int64_t n, int64_t lor_stride,
+                const float *tof_values, int64_t tof_stride,
+                const float *image, srf_grid_t grid, const srf_tof_t *tof,
+                float *out, void *stream);
+
+/* Backprojection (Backprojection / BackprojectionTof, siddon.cc:23-35,
+ * siddon_tof.cc:26-41; host fn backprojection(), siddon.cu.cc:731-753 and
+ * siddon_tof.cu.cc:788-812).  image[v] (+)= sum_e values[e] * w_ev.
+ * accumulate = 0 zeroes `image` first (the op's cudaMemset, siddon.cc:128);
+ * 1 adds into it.  `values` may be NULL, meaning all ones (the sensitivity
+ * image, emap_generator.py:169-201). */
+int srf_backproject(const float *lors, int64_t n, int64_t lor_stride,
+                    const float *tof_values, int64_t tof_stride,
+                    const float *values, srf_grid_t grid, const srf_tof_t *tof,
+                    srf_bp_mode_t mode, int accumulate,
+                    float *image, void *stream);
+
+/* Backprojection of the EM ratio q[e] = ratio(y[e], p[e]) without
+ * materialising q: the ratio of mlem.py:41-42 / mlem_full_tof.py:109 /
+ * osem_full.py:219-220 fused into the backprojector's event load. */
+int srf_backproject_ratio(const float *lors, int64_t n, int64_t lor_stride,
+                          const float *tof_values, int64_t tof_stride,
+                          const float *y, const float *p, srf_ratio_kind_t kind, float eps,
+                          srf_grid_t grid, const srf_tof_t *tof,
+                          srf_bp_mode_t mode, int accumulate,
+                          float *image, void *stream);
+
+/* q[e] = ratio(y[e], p[e]) (see srf_ratio_kind_t). */
+int srf_ratio(const float *y, const float *p, float *q, int64_t n,
+              srf_ratio_kind_t kind, float eps, void *stream);
+
+/* EM multiplicative update x <- x * bp / e, e = emap (zero_guard == 0,
+ * osem_full.py:221) or e = (emap == 0 ? zero_guard : emap) (mlem.py:33-35,43,
+ * zero_guard = 1e8).  16 B/voxel, one pass. */
+int srf_em_update(float *x, const float *bp, const float *emap, int64_t nvox,
+                  float zero_guard, void *stream);
+
+/* Per-event digest of the (voxel, branch) terms the projector
+ * (mode = SRF_BP_ADJOINT: projector predicates) or backprojector
+ * (SRF_BP_REFERENCE) visits: hash[e] = sum_terms mix64(voxel << 4 | branch),
+ * count[e] = number of terms.  Runs the same device marcher as srf_project /
+ * srf_backproject; used to prove voxel indexing bit-exact. */
+int srf_lor_digest(const float *lors, int64_t n, int64_t lor_stride,
+                   const float *tof_values, int64_t tof_stride,
+                   srf_grid_t grid, const srf_tof_t *tof, srf_bp_mode_t mode,
+                   uint64_t *hash, int32_t *count, void *stream);
+
+/* Layout conversions between the Python (nx,ny,nz) C-order image (z fastest)
+ * and the kernel layout (x fastest): dst = transpose(src). */
+int srf_image_to_kernel_layout(const float *src_xyz, float *dst_kernel, int32_t nx, int32_t ny,
+                               int32_t nz, void *stream);
+int srf_image_from_kernel_layout(const float *src_kernel, float *dst_xyz, int32_t nx, int32_t ny,
+                                 int32_t nz, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SRF_HIP_H */

@@ -1,0 +1,86 @@
+"""ctypes binding of libsrf_hip.so (include/srf_hip.h).
+
+The library is loaded after ``import torch`` so that it binds to the HIP
+runtime torch already loaded (both carry the SONAME libamdhip64.so.7): one
+runtime, so torch's stream handles and device pointers are valid here.
+
+There is no fallback: if the library is missing or fails to load, every op
+raises.  Build it with ``python -m srf_amd.build``.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the CDLL load, see above)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("SRF_HIP_LIB", os.path.join(_HERE, "libsrf_hip.so"))
+
+SRF_BP_REFERENCE = 0
+SRF_BP_ADJOINT = 1
+SRF_RATIO_PLAIN = 0
+SRF_RATIO_DIV_NO_NAN = 1
+
+
+class SrfGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("cx", ctypes.c_float), ("cy", ctypes.c_float), ("cz", ctypes.c_float),
+                ("sx", ctypes.c_float), ("sy", ctypes.c_float), ("sz", ctypes.c_float)]
+
+
+class SrfTof(ctypes.Structure):
+    _fields_ = [("tof_bin", ctypes.c_float), ("tof_sigma2", ctypes.c_float)]
+
+
+class SrfError(RuntimeError):
+    pass
+
+
+# (name, restype, argtypes) -- every symbol declared in include/srf_hip.h
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_F = ctypes.c_float
+_TOFP = ctypes.POINTER(SrfTof)
+SIGNATURES = [
+    ("srf_strerror", ctypes.c_char_p, [ctypes.c_int]),
+    ("srf_version", ctypes.c_int, []),
+    ("srf_project", ctypes.c_int, [_P, _I64, _I64, _P, _I64, _P, SrfGrid, _TOFP, _P, _P]),
+    ("srf_backproject", ctypes.c_int, [_P, _I64, _I64, _P, _I64, _P, SrfGrid, _TOFP, ctypes.c_int, ctypes.c_int,
+                                       _P, _P]),
+    ("srf_backproject_ratio", ctypes.c_int, [_P, _I64, _I64, _P, _I64, _P, _P, ctypes.c_int, _F, SrfGrid, _TOFP,
+                                             ctypes.c_int, ctypes.c_int, _P, _P]),
+    ("srf_ratio", ctypes.c_int, [_P, _P, _P, _I64, ctypes.c_int, _F, _P]),
+    ("srf_em_update", ctypes.c_int, [_P, _P, _P, _I64, _F, _P]),
+    ("srf_lor_digest", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.c_int, _P, _P, _P]),
+    ("srf_image_to_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
+    ("srf_image_from_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SrfError(
+                f"libsrf_hip.so not found at {LIB_PATH}; build it with `python -m srf_amd.build` "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().srf_strerror(rc).decode()
+        raise SrfError(f"{what} failed: {msg} ({rc})")
+
+
+def stream_handle(device=None):
+    """hipStream_t of torch's current stream on `device` (as an int)."""
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,169 @@
+// srf_elementwise.hip -- the EM loop's elementwise kernels, image layout
+// conversions and the ABI's small utilities.
+//
+// All of these are HBM-bound streaming kernels: 16-byte vector accesses,
+// grid sized to the data (no grid-stride), so a 32 MiB image is one pass.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "srf_hip.h"
+#include "srf_internal.hpp"
+
+namespace srf {
+
+__global__ __launch_bounds__(256) void k_fill(float *__restrict__ dst, float v, int64_t n)
+{
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i + 3 < n) {
+        *reinterpret_cast<float4 *>(dst + i) = make_float4(v, v, v, v);
+    } else {
+        for (int64_t j = i; j < n; ++j) dst[j] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_scalar(float *__restrict__ dst, float v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+int fill_f32(float *dst, float v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return SRF_OK;
+    if (!dst) return SRF_ERR_INVALID_ARG;
+    if ((reinterpret_cast<uintptr_t>(dst) & 15) != 0)
+        hipLaunchKernelGGL(k_fill_scalar, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, v, n);
+    else
+        hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 1023) / 1024)), dim3(256), 0, s, dst, v, n);
+    return launch_status();
+}
+
+// q = ratio(y, p)  (srf_ratio_kind_t)
+__global__ __launch_bounds__(256) void k_ratio(const float *__restrict__ y, const float *__restrict__ p,
+                                               float *__restrict__ q, int64_t n, int kind, float eps)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float d = p[i] + eps;
+    float r = y[i] / d;
+    if (kind == SRF_RATIO_DIV_NO_NAN && d == 0.0f) r = 0.0f;
+    q[i] = r;
+}
+
+// x <- x * bp / e   (ArithmeticMixin (x*bp)/emap, arithmetic_mixins.py:38-51)
+__global__ __launch_bounds__(256) void k_em_update(float *__restrict__ x, const float *__restrict__ bp,
+                                                   const float *__restrict__ emap, int64_t n, float guard)
+{
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i + 3 < n) {
+        float4 xv = *reinterpret_cast<const float4 *>(x + i);
+        const float4 bv = *reinterpret_cast<const float4 *>(bp + i);
+        float4 ev = *reinterpret_cast<const float4 *>(emap + i);
+        if (guard != 0.0f) {
+            ev.x = ev.x == 0.0f ? guard : ev.x;
+            ev.y = ev.y == 0.0f ? guard : ev.y;
+            ev.z = ev.z == 0.0f ? guard : ev.z;
+            ev.w = ev.w == 0.0f ? guard : ev.w;
+        }
+        xv.x = xv.x * bv.x / ev.x;
+        xv.y = xv.y * bv.y / ev.y;
+        xv.z = xv.z * bv.z / ev.z;
+        xv.w = xv.w * bv.w / ev.w;
+        *reinterpret_cast<float4 *>(x + i) = xv;
+    } else {
+        for (int64_t j = i; j < n; ++j) {
+            float e = emap[j];
+            if (guard != 0.0f && e == 0.0f) e = guard;
+            x[j] = x[j] * bp[j] / e;
+        }
+    }
+}
+
+// 2-D tiled transpose of each y-plane: dst[c][y][r] = src[r][y][c] for a
+// (R x Y x C) C-order source.  32x32 tiles staged through LDS (padded row) so
+// both the read and the write are coalesced.
+__global__ __launch_bounds__(256) void k_swap_outer(const float *__restrict__ src, float *__restrict__ dst, int R,
+                                                    int Y, int C)
+{
+    __shared__ float tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32, yy = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < R && c < C) tile[k][tx] = src[((int64_t)r * Y + yy) * C + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < R && c < C) dst[((int64_t)c * Y + yy) * R + r] = tile[tx][k];
+    }
+}
+
+static int swap_outer(const float *src, float *dst, int R, int Y, int C, hipStream_t s)
+{
+    if (!src || !dst || R <= 0 || Y <= 0 || C <= 0) return SRF_ERR_INVALID_ARG;
+    if (Y > 65535) return SRF_ERR_UNSUPPORTED;
+    dim3 grid((unsigned)((C + 31) / 32), (unsigned)((R + 31) / 32), (unsigned)Y);
+    hipLaunchKernelGGL(k_swap_outer, grid, dim3(256), 0, s, src, dst, R, Y, C);
+    return launch_status();
+}
+
+}  // namespace srf
+
+using namespace srf;
+
+extern "C" {
+
+const char *srf_strerror(int code)
+{
+    switch (code) {
+    case SRF_OK: return "ok";
+    case SRF_ERR_INVALID_ARG: return "invalid argument";
+    case SRF_ERR_UNSUPPORTED: return "unsupported size or configuration";
+    case SRF_ERR_LAUNCH: return "HIP launch or API call failed";
+    case SRF_ERR_WORKSPACE: return "workspace too small";
+    default: return "unknown error";
+    }
+}
+
+int srf_version(void) { return 1; }
+
+int srf_ratio(const float *y, const float *p, float *q, int64_t n, srf_ratio_kind_t kind, float eps, void *stream)
+{
+    if (n < 0) return SRF_ERR_INVALID_ARG;
+    if (n == 0) return SRF_OK;
+    if (!y || !p || !q) return SRF_ERR_INVALID_ARG;
+    if (kind != SRF_RATIO_PLAIN && kind != SRF_RATIO_DIV_NO_NAN) return SRF_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_ratio, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, y, p, q, n,
+                       (int)kind, eps);
+    return launch_status();
+}
+
+int srf_em_update(float *x, const float *bp, const float *emap, int64_t nvox, float zero_guard, void *stream)
+{
+    if (nvox < 0) return SRF_ERR_INVALID_ARG;
+    if (nvox == 0) return SRF_OK;
+    if (!x || !bp || !emap) return SRF_ERR_INVALID_ARG;
+    if (((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(bp) | reinterpret_cast<uintptr_t>(emap)) &
+         15) != 0)
+        return SRF_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_em_update, dim3((unsigned)((nvox + 1023) / 1024)), dim3(256), 0, (hipStream_t)stream, x, bp,
+                       emap, nvox, zero_guard);
+    return launch_status();
+}
+
+int srf_image_to_kernel_layout(const float *src_xyz, float *dst_kernel, int32_t nx, int32_t ny, int32_t nz,
+                               void *stream)
+{
+    // (nx, ny, nz) z-fastest  ->  (nz, ny, nx) x-fastest
+    return swap_outer(src_xyz, dst_kernel, nx, ny, nz, (hipStream_t)stream);
+}
+
+int srf_image_from_kernel_layout(const float *src_kernel, float *dst_xyz, int32_t nx, int32_t ny, int32_t nz,
+                                 void *stream)
+{
+    return swap_outer(src_kernel, dst_xyz, nz, ny, nx, (hipStream_t)stream);
+}
+
+}  // extern "C"

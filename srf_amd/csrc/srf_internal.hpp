@@ -1,0 +1,22 @@
+// srf_internal.hpp -- helpers shared by the translation units of libsrf_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "srf_hip.h"
+
+namespace srf {
+
+// Status of the launches just issued (hipGetLastError clears the sticky error
+// of a failed launch, so each entry point reports its own).
+inline int launch_status()
+{
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SRF_OK : SRF_ERR_LAUNCH;
+}
+
+// dst[i] = v for i < n.  A kernel, not hipMemsetAsync, so the call may be
+// captured into a HIP graph and replayed.
+int fill_f32(float *dst, float v, int64_t n, hipStream_t s);
+
+}  // namespace srf

@@ -1,0 +1,385 @@
+"""Drop-in operators and reconstruction drivers with srfnef's names and call
+signatures, running on the MI355X kernels of libsrf_hip.so.
+
+  Project(mode)(image, lors) -> Listmode              srfnef/functions/project.py:21-40
+  ProjectTof(mode, tof)(image, lors) -> Listmode      srfnef/functions/project_tof.py:22-47
+  BackProject(mode)(listmode, image) -> Image         srfnef/functions/back_project.py:22-49
+  BackProjectTof(mode, tof)(listmode, image) -> Image srfnef/functions/back_project_tof.py:21-52
+  Mlem(n_iter, emap)(listmode) -> Image               srfnef/functions/mlem.py:23-45
+  MlemFull(...)() / MlemFullTof(...)() -> Image       srfnef/functions/mlem_full.py, mlem_full_tof.py
+  OsemFull(...)() -> Image                            srfnef/functions/osem_full.py:30-87
+  OsemTof(...)() -> Image                             TOF ops in OsemFull's subset scheme
+  EmapGenerator(mode, scanner)(image_config) -> Emap  srfnef/functions/emap_generator.py:246-257
+
+``mode``: 'tf-eager' returns host numpy data (the reference's .numpy(),
+project.py:36); 'tf' and 'hip' return GPU tensors.  Every mode runs the HIP
+kernels -- there is no CPU path.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import ops
+from .data import Emap, Image, ImageConfig, Listmode, Lors, TofConfig, nef_class
+
+_MODES = ("hip", "tf-eager", "tf")
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("srf_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def to_gpu(a, dtype=torch.float32):
+    """numpy / tensor -> contiguous GPU tensor (no copy if already there)."""
+    if isinstance(a, torch.Tensor):
+        t = a
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(a)))
+    return t.to(device=device(), dtype=dtype).contiguous()
+
+
+def _out(t, mode):
+    if mode not in _MODES:
+        raise NotImplementedError(f"mode {mode!r}")
+    return t.detach().cpu().numpy() if mode == "tf-eager" else t
+
+
+def _geom(obj):
+    return [float(v) for v in obj.center] + [float(v) for v in obj.size]
+
+
+def _shape(obj):
+    return [int(v) for v in obj.shape]
+
+
+def image_to_kernel(data):
+    """(nx,ny,nz) -> GPU kernel layout (nz,ny,nx) (tf.transpose in project.py:27)."""
+    return ops.to_kernel_layout(to_gpu(data))
+
+
+def image_from_kernel(img_k):
+    return ops.from_kernel_layout(img_k)
+
+
+def _tof_args(tof):
+    if tof is None:
+        return False, 0.0, 0.0
+    return True, float(tof.tof_bin), float(tof.tof_sigma2)
+
+
+def project_data(image, lors_data, tof=None):
+    """GPU projection of an Image along (N,6|7) LOR data; returns a GPU tensor."""
+    use, tb, ts = _tof_args(tof)
+    return ops.project(to_gpu(lors_data), image_to_kernel(image.data), _shape(image), _geom(image), use, tb, ts)
+
+
+def backproject_data(values, lors_data, image, tof=None, adjoint=False):
+    """GPU backprojection into an image shaped like `image`; returns (nx,ny,nz) GPU tensor."""
+    use, tb, ts = _tof_args(tof)
+    v = None if values is None else to_gpu(values).reshape(-1)
+    img_k = ops.backproject(to_gpu(lors_data), v, _shape(image), _geom(image), use, tb, ts, int(adjoint))
+    return image_from_kernel(img_k)
+
+
+@nef_class
+class Project:
+    mode: str
+
+    def __call__(self, image: Image, lors: Lors) -> Listmode:
+        p = project_data(image, lors.data)
+        return Listmode(_out(p, self.mode), lors)
+
+
+@nef_class
+class ProjectTof:
+    mode: str
+    tof: TofConfig
+
+    def __call__(self, image: Image, lors: Lors) -> Listmode:
+        p = project_data(image, lors.data, tof=self.tof or TofConfig())
+        return Listmode(_out(p, self.mode), lors)
+
+
+@nef_class
+class BackProject:
+    mode: str
+
+    def __call__(self, listmode: Listmode, image: Image) -> Image:
+        # the input image only supplies shape/centre/size (siddon.cc:123-128)
+        bp = backproject_data(listmode.data, listmode.lors.data, image)
+        return image.update(data=_out(bp, self.mode))
+
+
+class BackProjectTof:
+    """A plain class, as in srfnef/functions/back_project_tof.py:21-24."""
+
+    def __init__(self, mode=None, tof=None):
+        self.mode = mode
+        self.tof = tof
+
+    def __call__(self, listmode: Listmode, image: Image) -> Image:
+        bp = backproject_data(listmode.data, listmode.lors.data, image, tof=self.tof or TofConfig())
+        return image.update(data=_out(bp, self.mode or "hip"))
+
+
+# --------------------------------------------------------------- EM engine --
+
+class EmEngine:
+    """Device-resident EM state for one scan: LORs, measured values, emap and
+    the image estimate x, all in kernel layout on the GPU.
+
+    One sub-iteration = project (srf_project) -> fused ratio+backproject
+    (srf_backproject_ratio) -> multiplicative update (srf_em_update)."""
+
+    def __init__(self, lors_data, values, emap_data, shape, geom, tof=None, ratio_kind=0, eps=0.0,
+                 zero_guard=0.0):
+        self.lors = to_gpu(lors_data)
+        self.y = to_gpu(values).reshape(-1)
+        self.shape = [int(v) for v in shape]
+        self.geom = [float(v) for v in geom]
+        self.use_tof, self.tof_bin, self.tof_sigma2 = _tof_args(tof)
+        self.emap_k = image_to_kernel(emap_data)
+        nx, ny, nz = self.shape
+        self.x = torch.ones((nz, ny, nx), dtype=torch.float32, device=self.lors.device)
+        self.ratio_kind = ratio_kind
+        self.eps = float(eps)
+        self.zero_guard = float(zero_guard)
+
+    def forward(self, lors=None):
+        lors = self.lors if lors is None else lors
+        return ops.project(lors, self.x, self.shape, self.geom, self.use_tof, self.tof_bin, self.tof_sigma2)
+
+    def sub_iteration(self, lors=None, y=None):
+        lors = self.lors if lors is None else lors
+        y = self.y if y is None else y
+        p = self.forward(lors)
+        bp = ops.backproject_ratio(lors, y, p, self.ratio_kind, self.eps, self.shape, self.geom, self.use_tof,
+                                   self.tof_bin, self.tof_sigma2, 0)
+        ops.em_update(self.x, bp, self.emap_k, self.zero_guard)
+
+    def image(self):
+        return image_from_kernel(self.x)
+
+
+@nef_class
+class Mlem:
+    n_iter: int
+    emap: Emap
+
+    def __call__(self, listmode: Listmode) -> Image:
+        # mlem.py:30-43: x0 = 1, emap 0 -> 1e8, q = y / (p + mean(y)*1e-8)
+        y = to_gpu(listmode.data).reshape(-1)
+        eps = float(np.float32(float(y.mean().item()) * 1e-8))
+        eng = EmEngine(listmode.lors.data, y, self.emap.data, self.emap.shape, _geom(self.emap),
+                       ratio_kind=0, eps=eps, zero_guard=1e8)
+        for _ in range(self.n_iter):
+            eng.sub_iteration()
+        return Image(eng.image().cpu().numpy(), list(self.emap.center), list(self.emap.size))
+
+
+def _mlem_full(self, tof):
+    if self.emap is None:
+        object.__setattr__(self, "emap", EmapGenerator("uniform", self.scanner)(self.image_config))
+    lm = self.listmode
+    if self.atten_corr is not None:
+        lm = self.atten_corr(lm)
+    # mlem_full.py:68 div_no_nan(y, p); mlem_full_tof.py:109 div_no_nan(y, p + 1e-5)
+    eps = 1e-5 if tof is not None else 0.0
+    eng = EmEngine(self.listmode.lors.data, lm.data, self.emap.data, self.image_config.shape,
+                   _geom(self.image_config), tof=tof, ratio_kind=1, eps=eps, zero_guard=1e8)
+    for _ in range(self.n_iter):
+        eng.sub_iteration()
+    x = Image(eng.image().cpu().numpy(), list(self.image_config.center), list(self.image_config.size))
+    if self.scatter_corr is not None:
+        assert self.atten_corr is not None
+        lm2 = self.scatter_corr(x, self.atten_corr.u_map, self.scanner, self.listmode)
+        eng = EmEngine(self.listmode.lors.data, lm2.data, self.emap.data, self.image_config.shape,
+                       _geom(self.image_config), tof=tof, ratio_kind=1, eps=eps, zero_guard=1e8)
+        for _ in range(self.n_iter):
+            eng.sub_iteration()
+        x = Image(eng.image().cpu().numpy(), list(self.image_config.center), list(self.image_config.size))
+    if self.psf_corr is not None:
+        x = self.psf_corr(x)
+    return x
+
+
+@nef_class
+class MlemFull:
+    n_iter: int
+    image_config: ImageConfig
+    scanner: object
+    listmode: Listmode
+    atten_corr: object
+    scatter_corr: object
+    psf_corr: object
+    emap: Emap
+
+    def __call__(self) -> Image:
+        return _mlem_full(self, None)
+
+
+@nef_class
+class MlemFullTof:
+    n_iter: int
+    image_config: ImageConfig
+    scanner: object
+    listmode: Listmode
+    atten_corr: object
+    scatter_corr: object
+    psf_corr: object
+    emap: Emap
+
+    def __call__(self) -> Image:
+        return _mlem_full(self, getattr(self.scanner, "tof", None) or TofConfig())
+
+
+def subset_permutation(n, seed):
+    """Seeded event permutation (the reference shuffles with the unseeded
+    global RNG, osem_full.py:207-208)."""
+    return np.random.default_rng(seed).permutation(n)
+
+
+class OsemEngine(EmEngine):
+    """OSEM over contiguous subsets of a once-permuted, device-resident LOR
+    array (no per-sub-iteration host gather or upload, cf. osem_full.py:213-217)."""
+
+    def __init__(self, lors_data, values, emap_data, shape, geom, n_sub, perm, tof=None, ratio_kind=0,
+                 eps=1e-8, zero_guard=0.0):
+        lors = to_gpu(lors_data)
+        y = to_gpu(values).reshape(-1)
+        n = lors.shape[0]
+        self.n_sub = int(n_sub)
+        self.n_per_sub = n // self.n_sub          # osem_full.py:209 (remainder dropped)
+        pidx = torch.as_tensor(np.asarray(perm[: self.n_per_sub * self.n_sub]), device=lors.device,
+                               dtype=torch.long)
+        super().__init__(lors.index_select(0, pidx).contiguous(), y.index_select(0, pidx).contiguous(),
+                         emap_data, shape, geom, tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard)
+        del lors, y
+
+    def subset(self, k):
+        a, b = k * self.n_per_sub, (k + 1) * self.n_per_sub
+        return self.lors[a:b], self.y[a:b]
+
+    def run(self, n_iter, reference_subset_order=False):
+        for i in range(n_iter):
+            k = (i // self.n_sub) if reference_subset_order else (i % self.n_sub)
+            lors, y = self.subset(k)
+            self.sub_iteration(lors, y)
+
+
+@nef_class
+class OsemFull:
+    n_iter: int
+    n_sub: int
+    image_config: ImageConfig
+    scanner: object
+    listmode: Listmode
+    atten_corr: object
+    scatter_corr: object
+    norm_corr: bool
+    psf_corr: object
+    emap: Emap
+    image: Image
+    seed: int = 0
+    reference_subset_order: bool = False
+
+    def __call__(self) -> Image:
+        if self.emap is None:
+            object.__setattr__(self, "emap", EmapGenerator("uniform", self.scanner)(self.image_config))
+        lm = self.listmode
+        if self.atten_corr is not None:
+            lm = self.atten_corr(lm)
+        perm = subset_permutation(len(self.listmode), self.seed)
+        # osem_full.py:219-221: q = y / (p + 1e-8), emap unguarded
+        eng = OsemEngine(self.listmode.lors.data, lm.data, self.emap.data, self.image_config.shape,
+                         _geom(self.image_config), self.n_sub, perm, ratio_kind=0, eps=1e-8, zero_guard=0.0)
+        eng.run(self.n_iter, self.reference_subset_order)
+        x = Image(eng.image().cpu().numpy(), list(self.image_config.center), list(self.image_config.size))
+        if self.psf_corr is not None:
+            x = self.psf_corr(x)
+        return x
+
+
+@nef_class
+class OsemTof:
+    """TOF list-mode OSEM: ProjectTof/BackProjectTof in OsemFull's subset loop,
+    seeded permutation, subset i % n_sub, MlemFullTof's div_no_nan(y, p+1e-5)
+    ratio and 1e8 emap guard."""
+    n_iter: int
+    n_sub: int
+    image_config: ImageConfig
+    tof: TofConfig
+    listmode: Listmode
+    atten_corr: object
+    emap: Emap
+    seed: int = 0
+
+    def __call__(self) -> Image:
+        lm = self.listmode
+        if self.atten_corr is not None:
+            lm = self.atten_corr(lm)
+        perm = subset_permutation(len(self.listmode), self.seed)
+        eng = OsemEngine(self.listmode.lors.data, lm.data, self.emap.data, self.image_config.shape,
+                         _geom(self.image_config), self.n_sub, perm, tof=self.tof or TofConfig(), ratio_kind=1,
+                         eps=1e-5, zero_guard=1e8)
+        eng.run(self.n_iter)
+        return Image(eng.image().cpu().numpy(), list(self.image_config.center), list(self.image_config.size))
+
+
+# ----------------------------------------------------------------- scanner --
+
+@nef_class
+class CylinderScanner:
+    """Ideal cylindrical detector surface (radius, axial length in mm) with an
+    optional TofConfig.  Stands in for srfnef's crystal-level scanner classes
+    (srfnef/geometry/pet_cylindrical_scanner.py:32-49) on synthetic data."""
+    inner_radius: float
+    axial_length: float
+    tof: TofConfig
+
+
+def uniform_cylinder_lors(n, radius, axial_length, seed, device_=None):
+    """n LORs with both endpoints uniform on the cylinder surface (torch, seeded)."""
+    g = torch.Generator(device="cpu").manual_seed(int(seed))
+    u = torch.rand((n, 4), generator=g, dtype=torch.float64)
+    phi1 = 2 * math.pi * u[:, 0]
+    phi2 = 2 * math.pi * u[:, 1]
+    z1 = (u[:, 2] - 0.5) * axial_length
+    z2 = (u[:, 3] - 0.5) * axial_length
+    lors = torch.stack([radius * torch.cos(phi1), radius * torch.sin(phi1), z1,
+                        radius * torch.cos(phi2), radius * torch.sin(phi2), z2], 1).to(torch.float32)
+    return lors if device_ is None else lors.to(device_)
+
+
+@nef_class
+class EmapGenerator:
+    """Sensitivity image = backprojection of ones over a LOR set.
+
+    The reference enumerates crystal pairs of a block scanner
+    (emap_generator.py:120-243).  For an ideal cylinder this builds the emap as
+    the backprojection of a seeded sample of uniform endpoint pairs (n_lors,
+    default 4e6), scaled by 1/n_lors -- the synthetic-data decision recorded
+    for the configs without crystals."""
+    mode: str
+    scanner: object
+    n_lors: int = 4_000_000
+    seed: int = 12345
+
+    def __call__(self, image_config) -> Emap:
+        sc = self.scanner
+        lors = uniform_cylinder_lors(int(self.n_lors), float(sc.inner_radius), float(sc.axial_length), self.seed,
+                                     device())
+        img = Image(None, list(image_config.center), list(image_config.size))
+        bp_k = ops.backproject(lors, None, [int(v) for v in image_config.shape], _geom(image_config), False, 0.0,
+                               0.0, 0)
+        emap = image_from_kernel(bp_k) / float(self.n_lors)
+        del img
+        return Emap(emap.cpu().numpy(), list(image_config.center), list(image_config.size))
+
+
+__all__ = ["Project", "ProjectTof", "BackProject", "BackProjectTof", "Mlem", "MlemFull", "MlemFullTof",
+           "OsemFull", "OsemTof", "EmapGenerator", "CylinderScanner", "EmEngine", "OsemEngine",
+           "uniform_cylinder_lors", "subset_permutation", "to_gpu", "project_data", "backproject_data"]
