@@ -42,11 +42,28 @@ def parse():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(nbytes):
+    """One plain device-to-device copy of `nbytes`: its FETCH_SIZE / WRITE_SIZE
+    counters calibrate the per-dispatch HBM traffic of the other kernels."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): every box here has one GPU; "
               "the event-sharded multi-GPU design is described in DESIGN.md", file=sys.stderr)
