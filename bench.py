@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--event-kernels", action="store_true",
+                    help="one-event-per-lane kernels instead of the LDS-tiled plans")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -104,9 +106,14 @@ def main():
     log(f"setup done in {time.time() - t0:.1f}s")
 
     perm = np.random.default_rng(args.seed).permutation(n)
+    tp = time.time()
     eng = OsemEngine(lors, y, emap, shape, geom, args.subsets, perm,
                      tof=TofConfig(tof_bin, tof_sigma2),
-                     ratio_kind=1, eps=1e-5, zero_guard=1e8)
+                     ratio_kind=1, eps=1e-5, zero_guard=1e8, tiled=not args.event_kernels)
+    torch.cuda.synchronize()
+    plan_s = time.time() - tp
+    plan_info = eng.plans[0].info() if eng.plans else None
+    log(f"engine ready in {plan_s:.1f}s; plan of subset 0: {plan_info}")
     del lors, y
     # exact term counts T_e per event (same marcher as the kernels; GPU digest)
     n_sub_ev = eng.n_per_sub
@@ -121,14 +128,20 @@ def main():
     stream = torch.cuda.current_stream()
     ev = []
 
-    def sub_iter(lors_s, y_s, timed):
+    def sub_iter(lors_s, y_s, plan, timed):
         if timed:
             a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
             a.record(stream)
-        p = ops.project(lors_s, eng.x, eng.shape, eng.geom, True, tof_bin, tof_sigma2)
+        if plan is not None:
+            p = plan.project(eng.x)
+        else:
+            p = ops.project(lors_s, eng.x, eng.shape, eng.geom, True, tof_bin, tof_sigma2)
         if timed:
             b.record(stream)
-        bp = ops.backproject_ratio(lors_s, y_s, p, 1, 1e-5, eng.shape, eng.geom, True, tof_bin, tof_sigma2, 0)
+        if plan is not None:
+            bp = plan.backproject_ratio(y_s, p, 1, 1e-5, 0)
+        else:
+            bp = ops.backproject_ratio(lors_s, y_s, p, 1, 1e-5, eng.shape, eng.geom, True, tof_bin, tof_sigma2, 0)
         if timed:
             c.record(stream)
             ev.append((a, b, c))
@@ -137,7 +150,7 @@ def main():
     def step(timed):
         for k in range(eng.n_sub):
             l_s, y_s = eng.subset(k)
-            sub_iter(l_s, y_s, timed)
+            sub_iter(l_s, y_s, eng.plans[k] if eng.plans else None, timed)
 
     for _ in range(args.warmup):
         step(False)
@@ -183,7 +196,9 @@ def main():
            "config": {"workload": "TOF list-mode OSEM, 256x256x128 x 2 mm, 5e7 events, 10 subsets, "
                                   "R 424 mm, axial 300 mm, TOF FWHM 60 mm (400 ps), water mu-map",
                       "events": n_used, "subsets": eng.n_sub, "image": list(shape), "voxel_mm": voxel,
-                      "tof_bin_mm": tof_bin, "tof_sigma2_mm2": tof_sigma2, "seed": args.seed},
+                      "tof_bin_mm": tof_bin, "tof_sigma2_mm2": tof_sigma2, "seed": args.seed,
+                      "kernels": "event-per-lane" if args.event_kernels else "lds-tiled plans",
+                      "setup_binning_s": plan_s, "plan_subset0": plan_info},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 

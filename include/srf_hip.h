@@ -3,8 +3,8 @@
  *
  * Conventions shared by every entry point
  *  - All pointers are device pointers (HBM) unless the comment says otherwise.
- *    The caller owns every buffer; nothing is allocated by the library except
- *    where `workspace` is documented as optional.
+ *    The caller owns every buffer; the library allocates device memory only in
+ *    srf_plan_create (freed by srf_plan_destroy).
  *  - LORs are AoS rows of `lor_stride` floats [x1,y1,z1,x2,y2,z2,(tof,...)] in
  *    mm, exactly Lors.data of srfnef/data/lors.py:12-22 (stride 6 or 7).
  *  - TOF values are read from `tof_values[e * tof_stride]`; pass
@@ -129,6 +129,40 @@ int srf_image_to_kernel_layout(const float *src_xyz, float *dst_kernel, int32_t 
                                int32_t nz, void *stream);
 int srf_image_from_kernel_layout(const float *src_kernel, float *dst_xyz, int32_t nx, int32_t ny,
                                  int32_t nz, void *stream);
+
+/* ---------------------------------------------------------------- plans --
+ * A plan bins a fixed event list (one OSEM subset, a sensitivity LOR set) into
+ * per-tile segments once, so that every later projection and backprojection of
+ * the same events runs the LDS-tiled kernels.  Same semantics and the same
+ * reference entry points as srf_project / srf_backproject above
+ * (siddon.cu.cc:706-753, siddon_tof.cu.cc:761-812); results agree with them to
+ * float round-off (summation order differs).
+ *
+ * srf_plan_create and srf_plan_destroy are the only entry points that allocate
+ * or free device memory (the binned records, about 64 B per event-tile segment,
+ * plus per-item tile buffers); srf_plan_info reports the total.  The LOR array
+ * is read during srf_plan_create only.  create synchronises `stream`. */
+typedef struct srf_plan srf_plan;
+
+int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride,
+                    const float *tof_values, int64_t tof_stride,
+                    srf_grid_t grid, const srf_tof_t *tof, srf_plan **plan, void *stream);
+int srf_plan_destroy(srf_plan *plan);
+int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments,
+                  int64_t *n_items, size_t *device_bytes);
+/* out[e] = projection of event e (srf_project semantics). */
+int srf_plan_project(srf_plan *plan, const float *image, float *out, void *stream);
+/* image (+)= backprojection (srf_backproject semantics; values NULL = ones). */
+int srf_plan_backproject(srf_plan *plan, const float *values, srf_bp_mode_t mode,
+                         int accumulate, float *image, void *stream);
+/* image (+)= backprojection of ratio(y, p) (srf_backproject_ratio semantics). */
+int srf_plan_backproject_ratio(srf_plan *plan, const float *y, const float *p,
+                               srf_ratio_kind_t kind, float eps, srf_bp_mode_t mode,
+                               int accumulate, float *image, void *stream);
+/* srf_lor_digest semantics, computed by the tiled kernels' own segment walk:
+ * proves that every term lands in exactly one tile segment. */
+int srf_plan_digest(srf_plan *plan, srf_bp_mode_t mode, uint64_t *hash, int32_t *count,
+                    void *stream);
 
 #ifdef __cplusplus
 }

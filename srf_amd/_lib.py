@@ -54,6 +54,15 @@ SIGNATURES = [
     ("srf_lor_digest", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.c_int, _P, _P, _P]),
     ("srf_image_to_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
     ("srf_image_from_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
+    ("srf_plan_create", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(_P), _P]),
+    ("srf_plan_destroy", ctypes.c_int, [_P]),
+    ("srf_plan_info", ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                                     ctypes.POINTER(ctypes.c_size_t)]),
+    ("srf_plan_project", ctypes.c_int, [_P, _P, _P, _P]),
+    ("srf_plan_backproject", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, _P, _P]),
+    ("srf_plan_backproject_ratio", ctypes.c_int, [_P, _P, _P, ctypes.c_int, _F, ctypes.c_int, ctypes.c_int, _P,
+                                                  _P]),
+    ("srf_plan_digest", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P]),
 ]
 
 _lib = None

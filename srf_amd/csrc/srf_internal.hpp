@@ -19,4 +19,8 @@ inline int launch_status()
 // captured into a HIP graph and replayed.
 int fill_f32(float *dst, float v, int64_t n, hipStream_t s);
 
+struct LaunchGeom;
+// Validates a grid (and TOF parameters) and derives the per-launch constants.
+int make_geom(const srf_grid_t &g, const srf_tof_t *tof, LaunchGeom &G);
+
 }  // namespace srf

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_digest(const float *__restrict__ lors, 
 
 // ----------------------------------------------------------------- host side --
 
-static int make_geom(const srf_grid_t &g, const srf_tof_t *tof, LaunchGeom &G)
+int make_geom(const srf_grid_t &g, const srf_tof_t *tof, LaunchGeom &G)
 {
     if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0) return SRF_ERR_INVALID_ARG;
     if ((int64_t)g.nx * g.ny * g.nz >= ((int64_t)1 << 31)) return SRF_ERR_UNSUPPORTED;

@@ -1,0 +1,734 @@
+// srf_tiled.hip -- LDS-tiled projector / backprojector over a binned event list
+// ("plan"), and its C-ABI entry points (include/srf_hip.h, srf_plan_*).
+//
+// Why: with one event per lane every projector term is a gather that misses
+// the per-XCD L2 (a 256x256x128 image is 32 MiB, an XCD's L2 4 MiB) and every
+// backprojector term is a device-scope atomic.  Here the image is cut into
+// TX x TY x TZ = 32 x 32 x 16 voxel tiles (64 KiB) and each event's slab range is
+// cut into segments, one per tile it can touch.  A work item is a run of
+// segments of one tile: the projector stages the tile in LDS once and every
+// term is an LDS read; the backprojector accumulates into an LDS tile (ds_add)
+// and writes it out once, with no global atomics.
+//
+// Binning (once per event list, reused by every iteration and both directions):
+//   k_bin_count  segments per event and per (tile, length class) key
+//   scan         event offsets (hipcub) and key offsets (host, <= 16384 keys)
+//   k_bin_emit   64-byte segment records, grouped by key: records of one tile
+//                are contiguous and sorted into slab-count classes so the lanes
+//                of a wavefront march similar lengths.
+// A record carries everything the marcher needs (the event's exact branch
+// parameters), so the passes stream records instead of gathering event data.
+//
+// Exactness: segments are a superset (value-space margin 1e-2 voxel, plus one
+// slab) of the slabs whose terms fall in the tile; inside, march_slabs runs the
+// reference's exact per-slab arithmetic and the sink keeps only the terms whose
+// cell lies in the tile, so every term of every event is applied exactly once.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "marcher.hpp"
+#include "srf_hip.h"
+#include "srf_internal.hpp"
+
+namespace srf {
+
+constexpr int TX = 32, TY = 32, TZ = 16;
+constexpr int TVOX = TX * TY * TZ;      // 16384 voxels, 64 KiB of LDS
+constexpr int NCLASS = 4;               // slab-count classes per tile
+constexpr int ITEM = 8192;              // records per work item (at most)
+constexpr int MAX_KEYS = 16384;         // LDS histogram of the binning kernels
+constexpr int BIN_THREADS = 256;
+constexpr int TILE_THREADS = 512;
+
+struct TileCfg {
+    int ntx, nty, ntz, ntiles, nkeys;
+};
+
+struct __align__(16) SegRec {
+    float p1, sec1, ks, z1;
+    float kz, Wbp, Wfp, ratio1;
+    float inv_pd, L;
+    int ev, slot;
+    uint32_t ab_bp, ab_fp;   // slab range a | b << 16 (backprojector / projector)
+    uint32_t meta;           // bit 0: x-dominant branch
+    uint32_t pad;
+};
+static_assert(sizeof(SegRec) == 64, "record is one 64-byte line");
+
+struct Item {
+    int start, count, tile, pad;
+};
+
+// ------------------------------------------------------------ enumeration --
+
+struct SegCtx {
+    bool fp_use, bp_use;   // this branch serves the projector / backprojector
+    int fa, fb;            // projector's exact cut range on this branch
+};
+
+// Calls f(tile, cls, a, b) for every segment of slabs [lo, hi) of branch B.
+template <class F>
+__device__ __forceinline__ void for_each_segment(const Branch &B, const RaySetup &r, const LaunchGeom &G,
+                                                 const TileCfg &T, int lo, int hi, F &&f)
+{
+    const int TP = B.xdom ? TX : TY;
+    const int TS = B.xdom ? TY : TX;
+    const int nts = B.xdom ? T.nty : T.ntx;
+    const double eps = 1e-2;
+    const double p1 = B.p1, np2 = B.np2;
+    for (int c = lo / TP; c * TP < hi; ++c) {
+        const int ca = max(lo, c * TP), cb = min(hi, (c + 1) * TP);
+        if (ca >= cb) continue;
+        const double xa = (double)ca - np2, xb = (double)cb - np2;
+        const double Sa = ((double)B.sec1 + (double)B.ks * (xa - p1)) / (double)B.ds + (double)B.ns2;
+        const double Sb = ((double)B.sec1 + (double)B.ks * (xb - p1)) / (double)B.ds + (double)B.ns2;
+        const double Za = ((double)r.z1 + (double)B.kz * (xa - p1)) / (double)G.dz + (double)G.nz2;
+        const double Zb = ((double)r.z1 + (double)B.kz * (xb - p1)) / (double)G.dz + (double)G.nz2;
+        const int ts0 = max(0, dbl_to_int(floor((fmin(Sa, Sb) - eps) / TS)));
+        const int ts1 = min(nts - 1, dbl_to_int(floor((fmax(Sa, Sb) + eps) / TS)));
+        const int tz0 = max(0, dbl_to_int(floor((fmin(Za, Zb) - eps) / TZ)));
+        const int tz1 = min(T.ntz - 1, dbl_to_int(floor((fmax(Za, Zb) + eps) / TZ)));
+        for (int ts = ts0; ts <= ts1; ++ts) {
+            double wa, wb;
+            axis_window((double)B.sec1, (double)B.ks, p1, (double)B.ds, (double)B.ns2, (double)(ts * TS) - eps,
+                        (double)((ts + 1) * TS) + eps, wa, wb);
+            if (wa > wb) continue;
+            const int sa = max(ca, dbl_to_int(floor(wa + np2)) - 2);
+            const int sb = min(cb, dbl_to_int(floor(wb + np2)) + 2);
+            if (sa >= sb) continue;
+            for (int tz = tz0; tz <= tz1; ++tz) {
+                double za, zb;
+                axis_window((double)r.z1, (double)B.kz, p1, (double)G.dz, (double)G.nz2, (double)(tz * TZ) - eps,
+                            (double)((tz + 1) * TZ) + eps, za, zb);
+                if (za > zb) continue;
+                const int a = max(sa, dbl_to_int(floor(za + np2)) - 2);
+                const int b = min(sb, dbl_to_int(floor(zb + np2)) + 2);
+                if (a >= b) continue;
+                const int tile = B.xdom ? c + T.ntx * (ts + T.nty * tz) : ts + T.ntx * (c + T.nty * tz);
+                const int cls = min(NCLASS - 1, ((b - a - 1) * NCLASS) / TP);
+                f(tile, cls, a, b);
+            }
+        }
+    }
+}
+
+// Every segment of one event: f(tile, cls, a, b, Branch, SegCtx).  Non-TOF
+// ties (|xd| == |yd|) march different branches in the projector (>=,
+// siddon.cu.cc:46) and the backprojector (>, :372): both are binned, each
+// marked for its own direction.
+template <bool TOF, class F>
+__device__ __forceinline__ void for_each_event_segment(const RaySetup &r, const LaunchGeom &G, const TileCfg &T,
+                                                       F &&f)
+{
+    if (!r.live) return;
+    const bool fp_live = TOF || !r.q2;
+    const bool xfp = TOF ? r.xdom_gt : r.xdom_ge;
+    const bool xbp = r.xdom_gt;
+    for (int pass = 0; pass < 2; ++pass) {
+        bool xdom;
+        SegCtx ctx;
+        if (pass == 0) {
+            xdom = xbp;
+            ctx.bp_use = true;
+            ctx.fp_use = fp_live && (xfp == xbp);
+        } else {
+            if (xfp == xbp || !fp_live) break;
+            xdom = xfp;
+            ctx.bp_use = false;
+            ctx.fp_use = true;
+        }
+        Branch B;
+        make_branch(r, xdom, G, B);
+        int lo, hi;
+        clip_range(B, r, G, lo, hi);
+        ctx.fa = lo;
+        ctx.fb = hi;
+        if (ctx.fp_use) fp_cut_range<TOF>(B, lo, hi, ctx.fa, ctx.fb);
+        int elo = ctx.bp_use ? lo : ctx.fa, ehi = ctx.bp_use ? hi : ctx.fb;
+        for_each_segment(B, r, G, T, elo, ehi, [&](int tile, int cls, int a, int b) { f(tile, cls, a, b, B, ctx); });
+    }
+}
+
+__device__ __forceinline__ uint32_t pack_ab(int a, int b) { return (uint32_t)a | ((uint32_t)b << 16); }
+
+template <bool TOF>
+__global__ __launch_bounds__(BIN_THREADS) void k_bin_count(const float *__restrict__ lors, int64_t n, int stride,
+                                                           const float *__restrict__ tofv, int tof_stride,
+                                                           LaunchGeom G, TileCfg T, int *__restrict__ ev_cnt,
+                                                           uint8_t *__restrict__ ev_flag,
+                                                           unsigned *__restrict__ key_count)
+{
+    extern __shared__ unsigned hist[];
+    for (int i = threadIdx.x; i < T.nkeys; i += BIN_THREADS) hist[i] = 0;
+    __syncthreads();
+    const int64_t e = (int64_t)blockIdx.x * BIN_THREADS + threadIdx.x;
+    if (e < n) {
+        RaySetup r;
+        ray_setup<TOF>(lors + e * stride, TOF ? tofv[e * tof_stride] : 0.0f, G, r);
+        int cnt = 0;
+        for_each_event_segment<TOF>(r, G, T, [&](int tile, int cls, int, int, const Branch &, const SegCtx &) {
+            atomicAdd(&hist[tile * NCLASS + cls], 1u);
+            ++cnt;
+        });
+        ev_cnt[e] = cnt;
+        ev_flag[e] = r.nan ? 1 : 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.nkeys; i += BIN_THREADS)
+        if (hist[i]) atomicAdd(&key_count[i], hist[i]);
+}
+
+template <bool TOF>
+__global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restrict__ lors, int64_t n, int stride,
+                                                          const float *__restrict__ tofv, int tof_stride,
+                                                          LaunchGeom G, TileCfg T, const int *__restrict__ ev_off,
+                                                          unsigned *__restrict__ key_cursor,
+                                                          SegRec *__restrict__ recs)
+{
+    extern __shared__ unsigned hist[];
+    for (int i = threadIdx.x; i < T.nkeys; i += BIN_THREADS) hist[i] = 0;
+    __syncthreads();
+    const int64_t e = (int64_t)blockIdx.x * BIN_THREADS + threadIdx.x;
+    RaySetup r;
+    const bool act = e < n;
+    if (act) {
+        ray_setup<TOF>(lors + e * stride, TOF ? tofv[e * tof_stride] : 0.0f, G, r);
+        for_each_event_segment<TOF>(r, G, T, [&](int tile, int cls, int, int, const Branch &, const SegCtx &) {
+            atomicAdd(&hist[tile * NCLASS + cls], 1u);
+        });
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.nkeys; i += BIN_THREADS)
+        if (hist[i]) hist[i] = atomicAdd(&key_cursor[i], hist[i]);
+    __syncthreads();
+    if (act) {
+        int j = ev_off[e];
+        for_each_event_segment<TOF>(r, G, T, [&](int tile, int cls, int a, int b, const Branch &B, const SegCtx &c) {
+            const unsigned pos = atomicAdd(&hist[tile * NCLASS + cls], 1u);
+            SegRec R;
+            R.p1 = B.p1;
+            R.sec1 = B.sec1;
+            R.ks = B.ks;
+            R.z1 = r.z1;
+            R.kz = B.kz;
+            R.Wbp = branch_weight<TOF>(B, r, G, false);
+            R.Wfp = branch_weight<TOF>(B, r, G, true);
+            R.ratio1 = r.ratio1;
+            R.inv_pd = TOF ? 1.0f / (B.p2 - B.p1) : 0.0f;
+            R.L = r.L;
+            R.ev = (int)e;
+            R.slot = j++;
+            R.ab_bp = c.bp_use ? pack_ab(a, b) : 0u;
+            const int fa = max(a, c.fa), fb = min(b, c.fb);
+            R.ab_fp = (c.fp_use && fa < fb) ? pack_ab(fa, fb) : 0u;
+            R.meta = B.xdom ? 1u : 0u;
+            R.pad = 0u;
+            recs[pos] = R;
+        });
+    }
+}
+
+// ------------------------------------------------------------ tile kernels --
+
+__device__ __forceinline__ void rec_branch(const SegRec &R, const LaunchGeom &G, Branch &B)
+{
+    B.xdom = (R.meta & 1u) != 0;
+    B.p1 = R.p1;
+    B.sec1 = R.sec1;
+    B.ks = R.ks;
+    B.kz = R.kz;
+    if (B.xdom) {
+        B.np = G.nx; B.ns = G.ny; B.np2 = G.nx2; B.ns2 = G.ny2;
+        B.pd_tof = 1.0f; B.ds = G.dy;
+    } else {
+        B.np = G.ny; B.ns = G.nx; B.np2 = G.ny2; B.ns2 = G.nx2;
+        B.pd_tof = G.dy; B.ds = 1.0f;
+    }
+}
+
+// Sink into an LDS tile stored x fastest: lds[lx + TX*(ly + TY*lz)].
+struct TileSink {
+    float *lds;
+    int p0, s0, z0;          // tile origin along primary, secondary, axial
+    int sp, ss;              // LDS strides of primary / secondary
+    float acc;               // projector accumulator
+    float value;             // backprojector value
+    __device__ __forceinline__ int slot(int ip, int cs, int cz, bool &in) const
+    {
+        const unsigned ds = (unsigned)(cs - s0), dz = (unsigned)(cz - z0);
+        in = ds < (unsigned)(sp == 1 ? TY : TX) && dz < (unsigned)TZ;
+        return (ip - p0) * sp + (int)ds * ss + (int)dz * (TX * TY);
+    }
+};
+
+struct TileGather : TileSink {
+    __device__ __forceinline__ void term(int ip, int cs, int cz, int, float w)
+    {
+        bool in;
+        const int i = slot(ip, cs, cz, in);
+        if (in) acc += lds[i] * w;
+    }
+};
+
+struct TileScatter : TileSink {
+    __device__ __forceinline__ void term(int ip, int cs, int cz, int, float w)
+    {
+        bool in;
+        const int i = slot(ip, cs, cz, in);
+        if (in) atomicAdd(lds + i, value * w);
+    }
+};
+
+__device__ __forceinline__ void tile_origin(int tile, const TileCfg &T, int &x0, int &y0, int &z0)
+{
+    x0 = (tile % T.ntx) * TX;
+    y0 = ((tile / T.ntx) % T.nty) * TY;
+    z0 = (tile / (T.ntx * T.nty)) * TZ;
+}
+
+template <class S>
+__device__ __forceinline__ void sink_for(S &s, float *lds, bool xdom, int x0, int y0, int z0)
+{
+    s.lds = lds;
+    s.z0 = z0;
+    if (xdom) { s.p0 = x0; s.s0 = y0; s.sp = 1; s.ss = TX; }
+    else { s.p0 = y0; s.s0 = x0; s.sp = TX; s.ss = 1; }
+}
+
+template <bool TOF, bool DS, bool DZ>
+__global__ __launch_bounds__(TILE_THREADS) void k_fp_tiled(const SegRec *__restrict__ recs,
+                                                           const Item *__restrict__ items,
+                                                           const float *__restrict__ img, LaunchGeom G, TileCfg T,
+                                                           float *__restrict__ partial)
+{
+    __shared__ float tile[TVOX];
+    const Item it = items[blockIdx.x];
+    int x0, y0, z0;
+    tile_origin(it.tile, T, x0, y0, z0);
+    for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) {
+        const int gx = x0 + (i % TX), gy = y0 + (i / TX) % TY, gz = z0 + i / (TX * TY);
+        tile[i] = (gx < G.nx && gy < G.ny && gz < G.nz) ? img[gx + G.nx * (gy + G.ny * gz)] : 0.0f;
+    }
+    __syncthreads();
+    for (int r = it.start + threadIdx.x; r < it.start + it.count; r += TILE_THREADS) {
+        const SegRec R = recs[r];
+        Branch B;
+        rec_branch(R, G, B);
+        TileGather s;
+        sink_for(s, tile, B.xdom, x0, y0, z0);
+        s.acc = 0.0f;
+        const TofLine TL{R.inv_pd, R.ratio1, R.L};
+        march_slabs<TOF, DS, DZ>(B, R.z1, G, (int)(R.ab_fp & 0xffffu), (int)(R.ab_fp >> 16), R.Wfp, TL, s);
+        partial[R.slot] = s.acc;
+    }
+}
+
+// value source: values[e] (values != null), ratio(y[e], p[e]) (y != null), else 1
+template <bool TOF, bool DS, bool DZ>
+__global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restrict__ recs,
+                                                           const Item *__restrict__ items,
+                                                           const float *__restrict__ values,
+                                                           const float *__restrict__ y, const float *__restrict__ p,
+                                                           int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
+                                                           float *__restrict__ tile_out)
+{
+    __shared__ float tile[TVOX];
+    const Item it = items[blockIdx.x];
+    int x0, y0, z0;
+    tile_origin(it.tile, T, x0, y0, z0);
+    for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) tile[i] = 0.0f;
+    __syncthreads();
+    for (int r = it.start + threadIdx.x; r < it.start + it.count; r += TILE_THREADS) {
+        const SegRec R = recs[r];
+        const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
+        const int a = (int)(ab & 0xffffu), b = (int)(ab >> 16);
+        if (a >= b) continue;
+        float v = 1.0f;
+        if (y) {
+            const float d = p[R.ev] + eps;
+            v = y[R.ev] / d;
+            if (kind == SRF_RATIO_DIV_NO_NAN && d == 0.0f) v = 0.0f;
+        } else if (values) {
+            v = values[R.ev];
+        }
+        Branch B;
+        rec_branch(R, G, B);
+        TileScatter s;
+        sink_for(s, tile, B.xdom, x0, y0, z0);
+        s.value = v;
+        const TofLine TL{R.inv_pd, R.ratio1, R.L};
+        march_slabs<TOF, DS, DZ>(B, R.z1, G, a, b, adjoint ? R.Wfp : R.Wbp, TL, s);
+    }
+    __syncthreads();
+    float *out = tile_out + (int64_t)blockIdx.x * TVOX;
+    for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = tile[i];
+}
+
+struct TileDigest : TileSink {
+    int gsp, gss, gsz;       // global strides of primary / secondary / axial
+    uint64_t h;
+    int cnt;
+    __device__ __forceinline__ void term(int ip, int cs, int cz, int br, float)
+    {
+        bool in;
+        (void)slot(ip, cs, cz, in);
+        if (in) {
+            const uint32_t v = (uint32_t)(ip * gsp + cs * gss + cz * gsz);
+            h += mix64(((uint64_t)v << 4) | (uint64_t)br);
+            cnt += 1;
+        }
+    }
+};
+
+template <bool TOF, bool DS, bool DZ>
+__global__ __launch_bounds__(TILE_THREADS) void k_digest_tiled(const SegRec *__restrict__ recs,
+                                                               const Item *__restrict__ items, int adjoint,
+                                                               LaunchGeom G, TileCfg T,
+                                                               unsigned long long *__restrict__ hash,
+                                                               int *__restrict__ count)
+{
+    const Item it = items[blockIdx.x];
+    int x0, y0, z0;
+    tile_origin(it.tile, T, x0, y0, z0);
+    for (int r = it.start + threadIdx.x; r < it.start + it.count; r += TILE_THREADS) {
+        const SegRec R = recs[r];
+        const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
+        Branch B;
+        rec_branch(R, G, B);
+        TileDigest d;
+        sink_for(d, nullptr, B.xdom, x0, y0, z0);
+        d.gsp = B.xdom ? 1 : G.nx;
+        d.gss = B.xdom ? G.nx : 1;
+        d.gsz = G.nx * G.ny;
+        d.h = 0;
+        d.cnt = 0;
+        const TofLine TL{R.inv_pd, R.ratio1, R.L};
+        march_slabs<TOF, DS, DZ>(B, R.z1, G, (int)(ab & 0xffffu), (int)(ab >> 16), 0.0f, TL, d);
+        if (d.cnt) {
+            atomicAdd(hash + R.ev, (unsigned long long)d.h);
+            atomicAdd(count + R.ev, d.cnt);
+        }
+    }
+}
+
+// image[v] (+)= sum over the work items of v's tile, in item order.
+__global__ __launch_bounds__(256) void k_tile_reduce(const float *__restrict__ tile_out,
+                                                     const int *__restrict__ tile_items, LaunchGeom G, TileCfg T,
+                                                     int accumulate, float *__restrict__ img)
+{
+    const int tile = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= TVOX) return;
+    int x0, y0, z0;
+    tile_origin(tile, T, x0, y0, z0);
+    const int gx = x0 + (i % TX), gy = y0 + (i / TX) % TY, gz = z0 + i / (TX * TY);
+    if (gx >= G.nx || gy >= G.ny || gz >= G.nz) return;
+    float s = 0.0f;
+    for (int k = tile_items[tile]; k < tile_items[tile + 1]; ++k) s += tile_out[(int64_t)k * TVOX + i];
+    const int64_t v = gx + (int64_t)G.nx * (gy + (int64_t)G.ny * gz);
+    img[v] = accumulate ? img[v] + s : s;
+}
+
+// p[e] = sum of e's segment partials (in segment order); NaN for non-finite LORs.
+__global__ __launch_bounds__(256) void k_seg_reduce(const float *__restrict__ partial, const int *__restrict__ ev_off,
+                                                    const uint8_t *__restrict__ ev_flag, int64_t n,
+                                                    float *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.0f;
+    for (int k = ev_off[e]; k < ev_off[e + 1]; ++k) s += partial[k];
+    out[e] = ev_flag[e] ? __builtin_nanf("") : s;
+}
+
+__global__ void k_fill_u32(unsigned *__restrict__ d, unsigned v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) d[i] = v;
+}
+
+}  // namespace srf
+
+using namespace srf;
+
+struct srf_plan {
+    LaunchGeom G;
+    TileCfg T;
+    bool tof, ds, dz;
+    int64_t n, nseg;
+    int nitems;
+    SegRec *recs;
+    Item *items;
+    int *tile_items;
+    int *ev_off;
+    uint8_t *ev_flag;
+    float *partial;
+    float *tile_buf;
+    size_t bytes;
+};
+
+namespace {
+
+template <class T>
+int dmalloc(T **p, size_t count, size_t &total)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return SRF_ERR_LAUNCH;
+    }
+    total += count * sizeof(T);
+    return SRF_OK;
+}
+
+void plan_free(srf_plan *P)
+{
+    if (!P) return;
+    (void)hipFree(P->recs);
+    (void)hipFree(P->items);
+    (void)hipFree(P->tile_items);
+    (void)hipFree(P->ev_off);
+    (void)hipFree(P->ev_flag);
+    (void)hipFree(P->partial);
+    (void)hipFree(P->tile_buf);
+    delete P;
+}
+
+}  // namespace
+
+#define SRF_TRY(x)                \
+    do {                          \
+        int rc__ = (x);           \
+        if (rc__) {               \
+            cleanup();            \
+            return rc__;          \
+        }                         \
+    } while (0)
+
+#define SRF_HIP(x)                                  \
+    do {                                            \
+        if ((x) != hipSuccess) {                    \
+            (void)hipGetLastError();                \
+            cleanup();                              \
+            return SRF_ERR_LAUNCH;                  \
+        }                                           \
+    } while (0)
+
+extern "C" {
+
+int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const float *tof_values, int64_t tof_stride,
+                    srf_grid_t grid, const srf_tof_t *tof, srf_plan **out, void *stream)
+{
+    if (!out) return SRF_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (n < 0 || (n > 0 && (!lors || lor_stride < 6))) return SRF_ERR_INVALID_ARG;
+    if (tof && n > 0 && (!tof_values || tof_stride < 1)) return SRF_ERR_INVALID_ARG;
+    if (n >= ((int64_t)1 << 31)) return SRF_ERR_UNSUPPORTED;
+    LaunchGeom G;
+    int rc = make_geom(grid, tof, G);
+    if (rc) return rc;
+    if (grid.nx > 65535 || grid.ny > 65535) return SRF_ERR_UNSUPPORTED;   // slab index packs in 16 bits
+    TileCfg T;
+    T.ntx = (grid.nx + TX - 1) / TX;
+    T.nty = (grid.ny + TY - 1) / TY;
+    T.ntz = (grid.nz + TZ - 1) / TZ;
+    T.ntiles = T.ntx * T.nty * T.ntz;
+    T.nkeys = T.ntiles * NCLASS;
+    if (T.nkeys > MAX_KEYS) return SRF_ERR_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+
+    srf_plan *P = new srf_plan();
+    memset(P, 0, sizeof(*P));
+    P->G = G;
+    P->T = T;
+    P->tof = tof != nullptr;
+    P->ds = G.dy != 1.0f;
+    P->dz = G.dz != 1.0f;
+    P->n = n;
+    unsigned *key_count = nullptr, *key_cursor = nullptr;
+    int *ev_cnt = nullptr;
+    void *scan_tmp = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(ev_cnt);
+        (void)hipFree(key_count);
+        (void)hipFree(key_cursor);
+        (void)hipFree(scan_tmp);
+        plan_free(P);
+    };
+    size_t scratch = 0;
+    SRF_TRY(dmalloc(&P->ev_off, (size_t)n + 1, P->bytes));
+    SRF_TRY(dmalloc(&P->ev_flag, (size_t)n, P->bytes));
+    SRF_TRY(dmalloc(&ev_cnt, (size_t)n + 1, scratch));
+    SRF_TRY(dmalloc(&key_count, (size_t)T.nkeys, scratch));
+    SRF_TRY(dmalloc(&key_cursor, (size_t)T.nkeys, scratch));
+    hipLaunchKernelGGL(k_fill_u32, dim3((T.nkeys + 255) / 256), dim3(256), 0, s, key_count, 0u, (int64_t)T.nkeys);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, (unsigned *)ev_cnt, 0u,
+                       n + 1);
+    const unsigned nb = (unsigned)((n + BIN_THREADS - 1) / BIN_THREADS);
+    const size_t shm = (size_t)T.nkeys * sizeof(unsigned);
+    if (n > 0) {
+        if (P->tof)
+            hipLaunchKernelGGL(k_bin_count<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
+                               tof_values, (int)tof_stride, G, T, ev_cnt, P->ev_flag, key_count);
+        else
+            hipLaunchKernelGGL(k_bin_count<false>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
+                               tof_values, (int)tof_stride, G, T, ev_cnt, P->ev_flag, key_count);
+    }
+    SRF_HIP(hipGetLastError());
+    // event offsets: exclusive scan of the per-event counts (ev_off[n] = total)
+    size_t tmp_bytes = 0;
+    SRF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ev_cnt, P->ev_off, (int)(n + 1), s));
+    SRF_HIP(hipMalloc(&scan_tmp, tmp_bytes ? tmp_bytes : 1));
+    SRF_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp_bytes, ev_cnt, P->ev_off, (int)(n + 1), s));
+    std::vector<unsigned> kc((size_t)T.nkeys);
+    int total = 0;
+    SRF_HIP(hipMemcpyAsync(kc.data(), key_count, kc.size() * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    SRF_HIP(hipMemcpyAsync(&total, P->ev_off + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    SRF_HIP(hipStreamSynchronize(s));
+    P->nseg = total;
+    // key offsets and work items (keys are tile-major, so a tile's items are contiguous)
+    std::vector<unsigned> koff((size_t)T.nkeys);
+    std::vector<Item> items;
+    std::vector<int> tile_items((size_t)T.ntiles + 1, 0);
+    unsigned acc = 0;
+    for (int k = 0; k < T.nkeys; ++k) {
+        koff[k] = acc;
+        const int tile = k / NCLASS;
+        if (k % NCLASS == 0) tile_items[tile] = (int)items.size();
+        for (unsigned st = 0; st < kc[k]; st += ITEM)
+            items.push_back(Item{(int)(acc + st), (int)std::min<unsigned>(ITEM, kc[k] - st), tile, 0});
+        acc += kc[k];
+    }
+    tile_items[T.ntiles] = (int)items.size();
+    if ((int64_t)acc != P->nseg) {
+        cleanup();
+        return SRF_ERR_LAUNCH;
+    }
+    P->nitems = (int)items.size();
+    SRF_TRY(dmalloc(&P->recs, (size_t)P->nseg, P->bytes));
+    SRF_TRY(dmalloc(&P->partial, (size_t)P->nseg, P->bytes));
+    SRF_TRY(dmalloc(&P->items, items.size(), P->bytes));
+    SRF_TRY(dmalloc(&P->tile_items, tile_items.size(), P->bytes));
+    SRF_TRY(dmalloc(&P->tile_buf, (size_t)std::max(1, P->nitems) * TVOX, P->bytes));
+    if (!items.empty())
+        SRF_HIP(hipMemcpyAsync(P->items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
+    SRF_HIP(hipMemcpyAsync(P->tile_items, tile_items.data(), tile_items.size() * sizeof(int), hipMemcpyHostToDevice,
+                           s));
+    SRF_HIP(hipMemcpyAsync(key_cursor, koff.data(), koff.size() * sizeof(unsigned), hipMemcpyHostToDevice, s));
+    if (n > 0) {
+        if (P->tof)
+            hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
+                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs);
+        else
+            hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
+                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs);
+    }
+    SRF_HIP(hipGetLastError());
+    SRF_HIP(hipStreamSynchronize(s));
+    (void)hipFree(ev_cnt);
+    (void)hipFree(key_count);
+    (void)hipFree(key_cursor);
+    (void)hipFree(scan_tmp);
+    *out = P;
+    return SRF_OK;
+}
+
+int srf_plan_destroy(srf_plan *plan)
+{
+    plan_free(plan);
+    return SRF_OK;
+}
+
+int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments, int64_t *n_items,
+                  size_t *device_bytes)
+{
+    if (!plan) return SRF_ERR_INVALID_ARG;
+    if (n_events) *n_events = plan->n;
+    if (n_segments) *n_segments = plan->nseg;
+    if (n_items) *n_items = plan->nitems;
+    if (device_bytes) *device_bytes = plan->bytes;
+    return SRF_OK;
+}
+
+#define PLAN_DISPATCH(KERNEL, ...)                                                                   \
+    do {                                                                                             \
+        if (P->tof) {                                                                                \
+            if (P->ds) { if (P->dz) KERNEL<true, true, true> __VA_ARGS__; else KERNEL<true, true, false> __VA_ARGS__; } \
+            else { if (P->dz) KERNEL<true, false, true> __VA_ARGS__; else KERNEL<true, false, false> __VA_ARGS__; } \
+        } else {                                                                                     \
+            if (P->ds) { if (P->dz) KERNEL<false, true, true> __VA_ARGS__; else KERNEL<false, true, false> __VA_ARGS__; } \
+            else { if (P->dz) KERNEL<false, false, true> __VA_ARGS__; else KERNEL<false, false, false> __VA_ARGS__; } \
+        }                                                                                            \
+    } while (0)
+
+int srf_plan_project(srf_plan *P, const float *image, float *out, void *stream)
+{
+    if (!P) return SRF_ERR_INVALID_ARG;
+    if (P->n == 0) return SRF_OK;
+    if (!image || !out) return SRF_ERR_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (P->nitems > 0)
+        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, image, P->G, P->T,
+                                                                       P->partial));
+    hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial, P->ev_off,
+                       P->ev_flag, P->n, out);
+    return launch_status();
+}
+
+static int plan_bp(srf_plan *P, const float *values, const float *y, const float *p, int kind, float eps,
+                   srf_bp_mode_t mode, int accumulate, float *image, hipStream_t s)
+{
+    if (!P || !image) return SRF_ERR_INVALID_ARG;
+    if (mode != SRF_BP_REFERENCE && mode != SRF_BP_ADJOINT) return SRF_ERR_INVALID_ARG;
+    if (P->nitems > 0)
+        PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, values, y, p, kind, eps,
+                                                                       (int)(mode == SRF_BP_ADJOINT), P->G, P->T,
+                                                                       P->tile_buf));
+    dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
+    hipLaunchKernelGGL(k_tile_reduce, grid, dim3(256), 0, s, P->tile_buf, P->tile_items, P->G, P->T, accumulate,
+                       image);
+    return launch_status();
+}
+
+int srf_plan_digest(srf_plan *P, srf_bp_mode_t mode, uint64_t *hash, int32_t *count, void *stream)
+{
+    if (!P) return SRF_ERR_INVALID_ARG;
+    if (mode != SRF_BP_REFERENCE && mode != SRF_BP_ADJOINT) return SRF_ERR_INVALID_ARG;
+    if (P->n == 0) return SRF_OK;
+    if (!hash || !count) return SRF_ERR_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned nb = (unsigned)((2 * P->n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(nb), dim3(256), 0, s, (unsigned *)hash, 0u, 2 * P->n);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, (unsigned *)count, 0u,
+                       P->n);
+    if (P->nitems > 0)
+        PLAN_DISPATCH(k_digest_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items,
+                                                                           (int)(mode == SRF_BP_ADJOINT), P->G,
+                                                                           P->T, (unsigned long long *)hash, count));
+    return launch_status();
+}
+
+int srf_plan_backproject(srf_plan *P, const float *values, srf_bp_mode_t mode, int accumulate, float *image,
+                         void *stream)
+{
+    return plan_bp(P, values, nullptr, nullptr, 0, 0.0f, mode, accumulate, image, (hipStream_t)stream);
+}
+
+int srf_plan_backproject_ratio(srf_plan *P, const float *y, const float *p, srf_ratio_kind_t kind, float eps,
+                               srf_bp_mode_t mode, int accumulate, float *image, void *stream)
+{
+    if (P && P->n > 0 && (!y || !p)) return SRF_ERR_INVALID_ARG;
+    if (kind != SRF_RATIO_PLAIN && kind != SRF_RATIO_DIV_NO_NAN) return SRF_ERR_INVALID_ARG;
+    return plan_bp(P, nullptr, y, p, (int)kind, eps, mode, accumulate, image, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -131,11 +131,13 @@ class EmEngine:
     """Device-resident EM state for one scan: LORs, measured values, emap and
     the image estimate x, all in kernel layout on the GPU.
 
-    One sub-iteration = project (srf_project) -> fused ratio+backproject
-    (srf_backproject_ratio) -> multiplicative update (srf_em_update)."""
+    One sub-iteration = project -> fused ratio+backproject -> multiplicative
+    update (srf_em_update).  With ``tiled`` (default) the event list is binned
+    once into an srf plan and both directions run the LDS-tiled kernels;
+    otherwise the one-event-per-lane kernels run on the raw LOR array."""
 
     def __init__(self, lors_data, values, emap_data, shape, geom, tof=None, ratio_kind=0, eps=0.0,
-                 zero_guard=0.0):
+                 zero_guard=0.0, tiled=True, plan=True):
         self.lors = to_gpu(lors_data)
         self.y = to_gpu(values).reshape(-1)
         self.shape = [int(v) for v in shape]
@@ -147,17 +149,32 @@ class EmEngine:
         self.ratio_kind = ratio_kind
         self.eps = float(eps)
         self.zero_guard = float(zero_guard)
+        self.tiled = bool(tiled)
+        self.plan = self.make_plan(self.lors) if (self.tiled and plan) else None
 
-    def forward(self, lors=None):
+    def make_plan(self, lors):
+        return ops.Plan(lors, self.shape, self.geom, self.use_tof, self.tof_bin, self.tof_sigma2)
+
+    def forward(self, lors=None, plan=None):
+        if plan is None and lors is None:
+            plan = self.plan
+        if plan is not None:
+            return plan.project(self.x)
         lors = self.lors if lors is None else lors
         return ops.project(lors, self.x, self.shape, self.geom, self.use_tof, self.tof_bin, self.tof_sigma2)
 
-    def sub_iteration(self, lors=None, y=None):
-        lors = self.lors if lors is None else lors
+    def sub_iteration(self, lors=None, y=None, plan=None):
+        if plan is None and lors is None:
+            plan = self.plan
         y = self.y if y is None else y
-        p = self.forward(lors)
-        bp = ops.backproject_ratio(lors, y, p, self.ratio_kind, self.eps, self.shape, self.geom, self.use_tof,
-                                   self.tof_bin, self.tof_sigma2, 0)
+        if plan is not None:
+            p = plan.project(self.x)
+            bp = plan.backproject_ratio(y, p, self.ratio_kind, self.eps, 0)
+        else:
+            lors = self.lors if lors is None else lors
+            p = ops.project(lors, self.x, self.shape, self.geom, self.use_tof, self.tof_bin, self.tof_sigma2)
+            bp = ops.backproject_ratio(lors, y, p, self.ratio_kind, self.eps, self.shape, self.geom, self.use_tof,
+                                       self.tof_bin, self.tof_sigma2, 0)
         ops.em_update(self.x, bp, self.emap_k, self.zero_guard)
 
     def image(self):
@@ -244,10 +261,11 @@ def subset_permutation(n, seed):
 
 class OsemEngine(EmEngine):
     """OSEM over contiguous subsets of a once-permuted, device-resident LOR
-    array (no per-sub-iteration host gather or upload, cf. osem_full.py:213-217)."""
+    array (no per-sub-iteration host gather or upload, cf. osem_full.py:213-217);
+    with ``tiled`` each subset is binned into its own plan at construction."""
 
     def __init__(self, lors_data, values, emap_data, shape, geom, n_sub, perm, tof=None, ratio_kind=0,
-                 eps=1e-8, zero_guard=0.0):
+                 eps=1e-8, zero_guard=0.0, tiled=True):
         lors = to_gpu(lors_data)
         y = to_gpu(values).reshape(-1)
         n = lors.shape[0]
@@ -256,8 +274,10 @@ class OsemEngine(EmEngine):
         pidx = torch.as_tensor(np.asarray(perm[: self.n_per_sub * self.n_sub]), device=lors.device,
                                dtype=torch.long)
         super().__init__(lors.index_select(0, pidx).contiguous(), y.index_select(0, pidx).contiguous(),
-                         emap_data, shape, geom, tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard)
+                         emap_data, shape, geom, tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard,
+                         tiled=tiled, plan=False)
         del lors, y
+        self.plans = [self.make_plan(self.subset(k)[0]) for k in range(self.n_sub)] if self.tiled else None
 
     def subset(self, k):
         a, b = k * self.n_per_sub, (k + 1) * self.n_per_sub
@@ -267,7 +287,10 @@ class OsemEngine(EmEngine):
         for i in range(n_iter):
             k = (i // self.n_sub) if reference_subset_order else (i % self.n_sub)
             lors, y = self.subset(k)
-            self.sub_iteration(lors, y)
+            if self.plans is not None:
+                self.sub_iteration(y=y, plan=self.plans[k])
+            else:
+                self.sub_iteration(lors, y)
 
 
 @nef_class

@@ -248,5 +248,135 @@ def _(img_k):
     return img_k.new_empty((nx, ny, nz))
 
 
-__all__ = ["project", "backproject", "backproject_ratio", "ratio", "em_update", "lor_digest",
+# -------------------------------------------------------------------- plans --
+
+class Plan:
+    """A binned event list (srf_plan_*): the LDS-tiled projector/backprojector.
+
+    Built once per fixed event list (an OSEM subset, a sensitivity LOR set) and
+    reused by every iteration; results equal the one-shot ops above to float
+    round-off.  Device memory is owned by the plan and freed by close()/GC."""
+
+    def __init__(self, lors: Tensor, shape, geom, use_tof: bool, tof_bin: float = 0.0, tof_sigma2: float = 0.0,
+                 tof_values: Optional[Tensor] = None):
+        n, stride, tp, ts = _lor_args(lors, tof_values, use_tof)
+        self.shape = [int(v) for v in shape]
+        self.geom = [float(v) for v in geom]
+        self.use_tof, self.tof_bin, self.tof_sigma2 = bool(use_tof), float(tof_bin), float(tof_sigma2)
+        self.n = n
+        self.device = lors.device
+        self._h = ctypes.c_void_p()
+        with torch.cuda.device(lors.device):
+            rc = lib().srf_plan_create(lors.data_ptr(), n, stride, tp, ts, _grid(self.shape, self.geom),
+                                       _tof(use_tof, tof_bin, tof_sigma2), ctypes.byref(self._h),
+                                       stream_handle(lors.device))
+        check(rc, "srf_plan_create")
+        self._lib = lib()
+
+    @property
+    def handle(self) -> int:
+        if not self._h:
+            raise ValueError("plan is closed")
+        return int(self._h.value)
+
+    def info(self):
+        ne, ns, ni = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        nb = ctypes.c_size_t()
+        check(lib().srf_plan_info(self.handle, ctypes.byref(ne), ctypes.byref(ns), ctypes.byref(ni),
+                                  ctypes.byref(nb)), "srf_plan_info")
+        return {"events": ne.value, "segments": ns.value, "items": ni.value, "device_bytes": nb.value}
+
+    def project(self, image_k: Tensor) -> Tensor:
+        return plan_project(self.handle, image_k, self.n, self.shape)
+
+    def backproject(self, values: Optional[Tensor] = None, mode: int = 0) -> Tensor:
+        return plan_backproject(self.handle, values, self.n, self.shape, int(mode))
+
+    def backproject_ratio(self, y: Tensor, p: Tensor, kind: int, eps: float, mode: int = 0) -> Tensor:
+        return plan_backproject_ratio(self.handle, y, p, int(kind), float(eps), self.n, self.shape, int(mode))
+
+    def digest(self, mode: int):
+        h = torch.empty(self.n, dtype=torch.int64, device=self.device)
+        c = torch.empty(self.n, dtype=torch.int32, device=self.device)
+        with torch.cuda.device(self.device):
+            rc = lib().srf_plan_digest(self.handle, int(mode), h.data_ptr(), c.data_ptr(),
+                                       stream_handle(self.device))
+        check(rc, "srf_plan_digest")
+        return h, c
+
+    def close(self):
+        if self._h:
+            torch.cuda.synchronize(self.device)
+            self._lib.srf_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@torch.library.custom_op("srf::plan_project", mutates_args=())
+def plan_project(handle: int, image_k: Tensor, n: int, shape: List[int]) -> Tensor:
+    """Projection of every event of a plan (srf_plan_project)."""
+    _check_image(image_k, shape)
+    out = torch.empty(n, dtype=torch.float32, device=image_k.device)
+    with torch.cuda.device(image_k.device):
+        rc = lib().srf_plan_project(handle, image_k.data_ptr(), out.data_ptr(), stream_handle(image_k.device))
+    check(rc, "srf_plan_project")
+    return out
+
+
+@plan_project.register_fake
+def _(handle, image_k, n, shape):
+    return image_k.new_empty(n)
+
+
+@torch.library.custom_op("srf::plan_backproject", mutates_args=())
+def plan_backproject(handle: int, values: Optional[Tensor], n: int, shape: List[int], mode: int) -> Tensor:
+    nx, ny, nz = (int(v) for v in shape)
+    dev = values.device if values is not None else torch.device("cuda", torch.cuda.current_device())
+    vp = None
+    if values is not None:
+        _req(values, "values")
+        if values.numel() != n:
+            raise ValueError("values must have one entry per event")
+        vp = values.data_ptr()
+    img = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        rc = lib().srf_plan_backproject(handle, vp, int(mode), 0, img.data_ptr(), stream_handle(dev))
+    check(rc, "srf_plan_backproject")
+    return img
+
+
+@plan_backproject.register_fake
+def _(handle, values, n, shape, mode):
+    nx, ny, nz = shape
+    return torch.empty((nz, ny, nx), device="meta")
+
+
+@torch.library.custom_op("srf::plan_backproject_ratio", mutates_args=())
+def plan_backproject_ratio(handle: int, y: Tensor, p: Tensor, kind: int, eps: float, n: int, shape: List[int],
+                           mode: int) -> Tensor:
+    _req(y, "y")
+    _req(p, "p")
+    if y.numel() != n or p.numel() != n:
+        raise ValueError("y and p must have one entry per event")
+    nx, ny, nz = (int(v) for v in shape)
+    img = torch.empty((nz, ny, nx), dtype=torch.float32, device=y.device)
+    with torch.cuda.device(y.device):
+        rc = lib().srf_plan_backproject_ratio(handle, y.data_ptr(), p.data_ptr(), int(kind), float(eps), int(mode), 0,
+                                              img.data_ptr(), stream_handle(y.device))
+    check(rc, "srf_plan_backproject_ratio")
+    return img
+
+
+@plan_backproject_ratio.register_fake
+def _(handle, y, p, kind, eps, n, shape, mode):
+    nx, ny, nz = shape
+    return y.new_empty((nz, ny, nx))
+
+
+__all__ = ["Plan", "plan_project", "plan_backproject", "plan_backproject_ratio", "project", "backproject", "backproject_ratio", "ratio", "em_update", "lor_digest",
            "to_kernel_layout", "from_kernel_layout", "_lib"]
