@@ -19,8 +19,8 @@
 // A record carries everything the marcher needs (the event's exact branch
 // parameters), so the passes stream records instead of gathering event data.
 //
-// Exactness: segments are a superset (value-space margin 1e-2 voxel, plus one
-// slab) of the slabs whose terms fall in the tile; inside, march_slabs runs the
+// Exactness: segments are a superset (value-space margin 1e-2 voxel, far above
+// float round-off) of the slabs whose terms fall in the tile; inside, march_slabs runs the
 // reference's exact per-slab arithmetic and the sink keeps only the terms whose
 // cell lies in the tile, so every term of every event is applied exactly once.
 #include <hip/hip_runtime.h>
@@ -45,7 +45,7 @@ constexpr int NCLASS = 4;               // slab-count classes per tile
 constexpr int ITEM = 8192;              // records per work item (at most)
 constexpr int MAX_KEYS = 16384;         // LDS histogram of the binning kernels
 constexpr int BIN_THREADS = 256;
-constexpr int TILE_THREADS = 512;
+constexpr int TILE_THREADS = 1024;
 
 struct TileCfg {
     int ntx, nty, ntz, ntiles, nkeys;
@@ -53,12 +53,12 @@ struct TileCfg {
 
 struct __align__(16) SegRec {
     float p1, sec1, ks, z1;
-    float kz, Wbp, Wfp, ratio1;
-    float inv_pd, L;
+    float kz, Wbp, Wfp;
+    float tA, tB;            // TOF: exp(-d^2 / (2 tse)) == exp2(-(pp1 * tA + tB)^2), pp1 = ip - np2
     int ev, slot;
     uint32_t ab_bp, ab_fp;   // slab range a | b << 16 (backprojector / projector)
     uint32_t meta;           // bit 0: x-dominant branch
-    uint32_t pad;
+    uint32_t pad[2];
 };
 static_assert(sizeof(SegRec) == 64, "record is one 64-byte line");
 
@@ -100,16 +100,17 @@ __device__ __forceinline__ void for_each_segment(const Branch &B, const RaySetup
             axis_window((double)B.sec1, (double)B.ks, p1, (double)B.ds, (double)B.ns2, (double)(ts * TS) - eps,
                         (double)((ts + 1) * TS) + eps, wa, wb);
             if (wa > wb) continue;
-            const int sa = max(ca, dbl_to_int(floor(wa + np2)) - 2);
-            const int sb = min(cb, dbl_to_int(floor(wb + np2)) + 2);
+            // slabs [ip - np2, ip - np2 + 1] meeting [wa, wb]
+            const int sa = max(ca, dbl_to_int(ceil(wa + np2)) - 1);
+            const int sb = min(cb, dbl_to_int(floor(wb + np2)) + 1);
             if (sa >= sb) continue;
             for (int tz = tz0; tz <= tz1; ++tz) {
                 double za, zb;
                 axis_window((double)r.z1, (double)B.kz, p1, (double)G.dz, (double)G.nz2, (double)(tz * TZ) - eps,
                             (double)((tz + 1) * TZ) + eps, za, zb);
                 if (za > zb) continue;
-                const int a = max(sa, dbl_to_int(floor(za + np2)) - 2);
-                const int b = min(sb, dbl_to_int(floor(zb + np2)) + 2);
+                const int a = max(sa, dbl_to_int(ceil(za + np2)) - 1);
+                const int b = min(sb, dbl_to_int(floor(zb + np2)) + 1);
                 if (a >= b) continue;
                 const int tile = B.xdom ? c + T.ntx * (ts + T.nty * tz) : ts + T.ntx * (c + T.nty * tz);
                 const int cls = min(NCLASS - 1, ((b - a - 1) * NCLASS) / TP);
@@ -157,6 +158,25 @@ __device__ __forceinline__ void for_each_event_segment(const RaySetup &r, const 
 }
 
 __device__ __forceinline__ uint32_t pack_ab(int a, int b) { return (uint32_t)a | ((uint32_t)b << 16); }
+
+// TOF Gaussian of siddon_tof.cu.cc:55-59 as a linear function of the slab's
+// lower boundary pp1 = ip - np2 (exact in float):
+//   pc = ((ip - np/2) + 0.5) * pd_tof = (pp1 + off) * pd_tof, off = np2 - np/2 + 0.5
+//   d  = ((pc - p1) / (p2 - p1) - ratio1) * L
+//   exp(-d^2 / (2 tse)) = exp2(-(d * sqrt(log2(e) / (2 tse)))^2)
+// Coefficients in double, rounded once.
+template <bool TOF>
+__device__ __forceinline__ void tof_coeffs(const Branch &B, const RaySetup &r, const LaunchGeom &G, float &tA,
+                                           float &tB)
+{
+    tA = tB = 0.0f;
+    if (!TOF) return;
+    const double off = (double)B.np2 - (double)(B.np / 2) + 0.5;
+    const double s = sqrt(0.5 * 1.4426950408889634 / (double)G.tse);
+    const double k = (double)r.L / ((double)B.p2 - (double)B.p1);
+    tA = (float)((double)B.pd_tof * k * s);
+    tB = (float)(((off * (double)B.pd_tof - (double)B.p1) * k - (double)r.ratio1 * (double)r.L) * s);
+}
 
 template <bool TOF>
 __global__ __launch_bounds__(BIN_THREADS) void k_bin_count(const float *__restrict__ lors, int64_t n, int stride,
@@ -220,16 +240,14 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
             R.kz = B.kz;
             R.Wbp = branch_weight<TOF>(B, r, G, false);
             R.Wfp = branch_weight<TOF>(B, r, G, true);
-            R.ratio1 = r.ratio1;
-            R.inv_pd = TOF ? 1.0f / (B.p2 - B.p1) : 0.0f;
-            R.L = r.L;
+            tof_coeffs<TOF>(B, r, G, R.tA, R.tB);
             R.ev = (int)e;
             R.slot = j++;
             R.ab_bp = c.bp_use ? pack_ab(a, b) : 0u;
             const int fa = max(a, c.fa), fb = min(b, c.fb);
             R.ab_fp = (c.fp_use && fa < fb) ? pack_ab(fa, fb) : 0u;
             R.meta = B.xdom ? 1u : 0u;
-            R.pad = 0u;
+            R.pad[0] = R.pad[1] = 0u;
             recs[pos] = R;
         });
     }
@@ -237,54 +255,139 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
 
 // ------------------------------------------------------------ tile kernels --
 
-__device__ __forceinline__ void rec_branch(const SegRec &R, const LaunchGeom &G, Branch &B)
+// v_cvt_flr_i32_f32: floor and convert in one VALU op (NaN -> 0, saturating,
+// as v_floor_f32 + v_cvt_i32_f32).
+__device__ __forceinline__ int cvt_flr(float v)
 {
-    B.xdom = (R.meta & 1u) != 0;
-    B.p1 = R.p1;
-    B.sec1 = R.sec1;
-    B.ks = R.ks;
-    B.kz = R.kz;
-    if (B.xdom) {
-        B.np = G.nx; B.ns = G.ny; B.np2 = G.nx2; B.ns2 = G.ny2;
-        B.pd_tof = 1.0f; B.ds = G.dy;
-    } else {
-        B.np = G.ny; B.ns = G.nx; B.np2 = G.ny2; B.ns2 = G.nx2;
-        B.pd_tof = G.dy; B.ds = 1.0f;
-    }
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
 }
 
-// Sink into an LDS tile stored x fastest: lds[lx + TX*(ly + TY*lz)].
-struct TileSink {
+enum : int { M_GATHER = 0, M_SCATTER = 1, M_DIGEST = 2 };
+
+// State of one segment walk inside one LDS tile (stored x fastest:
+// lds[lx + TX*(ly + TY*lz)]).  Terms whose cell lies outside the tile, or that
+// the reference's predicates reject, are sent to `dummy`: a zero word for
+// gathers, a per-lane scratch word for scatters, so the slab body has no
+// branches and its three LDS accesses issue back to back.
+struct TileWalk {
     float *lds;
-    int p0, s0, z0;          // tile origin along primary, secondary, axial
-    int sp, ss;              // LDS strides of primary / secondary
-    float acc;               // projector accumulator
-    float value;             // backprojector value
-    __device__ __forceinline__ int slot(int ip, int cs, int cz, bool &in) const
-    {
-        const unsigned ds = (unsigned)(cs - s0), dz = (unsigned)(cz - z0);
-        in = ds < (unsigned)(sp == 1 ? TY : TX) && dz < (unsigned)TZ;
-        return (ip - p0) * sp + (int)ds * ss + (int)dz * (TX * TY);
-    }
+    int dummy;
+    int p0, s0, z0;    // tile origin along primary, secondary, axial
+    int sp, ss;        // LDS strides of primary / secondary
+    float acc;         // gather accumulator
+    float value;       // scatter value
+    uint64_t h;        // digest
+    int cnt;
+    int gsp, gss, gsz; // global strides (digest)
 };
 
-struct TileGather : TileSink {
-    __device__ __forceinline__ void term(int ip, int cs, int cz, int, float w)
+template <bool TOF, bool DS, bool DZ, int MODE>
+__device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G, int a, int b, float W,
+                                           TileWalk &t)
+{
+    const bool xdom = (R.meta & 1u) != 0;
+    const float p1 = R.p1, sec1 = R.sec1, ks = R.ks, kz = R.kz, z1 = R.z1;
+    const float ds = xdom ? G.dy : 1.0f;
+    const float ns2 = xdom ? G.ny2 : G.nx2;
+    const float np2 = xdom ? G.nx2 : G.ny2;
+    const int ns = xdom ? G.ny : G.nx, nz = G.nz;
+    const bool ks_pos = ks >= 0.0f, kz_pos = kz >= 0.0f;
+    float pp1 = (float)a - np2;
+    float S0, Z0;
     {
-        bool in;
-        const int i = slot(ip, cs, cz, in);
-        if (in) acc += lds[i] * w;
+        const float sv = sec1 + ks * (pp1 - p1);
+        S0 = (DS ? sv / ds : sv) + ns2;
+        const float zv = z1 + kz * (pp1 - p1);
+        Z0 = (DZ ? zv / G.dz : zv) + G.nz2;
     }
-};
-
-struct TileScatter : TileSink {
-    __device__ __forceinline__ void term(int ip, int cs, int cz, int, float w)
-    {
-        bool in;
-        const int i = slot(ip, cs, cz, in);
-        if (in) atomicAdd(lds + i, value * w);
+    int base = (a - t.p0) * t.sp;
+    for (int ip = a; ip < b; ++ip) {
+        const float pp2 = pp1 + 1.0f;
+        float S1, Z1;
+        {
+            const float sv = sec1 + ks * (pp2 - p1);
+            S1 = (DS ? sv / ds : sv) + ns2;
+            const float zv = z1 + kz * (pp2 - p1);
+            Z1 = (DZ ? zv / G.dz : zv) + G.nz2;
+        }
+        const float sl = ks_pos ? S0 : S1, sh = ks_pos ? S1 : S0;
+        const float zl = kz_pos ? Z0 : Z1, zh = kz_pos ? Z1 : Z0;
+        const int cs1 = cvt_flr(sl), cs2 = cvt_flr(sh);
+        const int cz1 = cvt_flr(zl), cz2 = cvt_flr(zh);
+        const bool bs = cs1 != cs2, bz = cz1 != cz2;
+        // top-level tests of siddon.cu.cc:86-189 (x) / 231-333 (y)
+        const bool ok = cs1 >= (bs ? -1 : 0) && cs1 < ns && cz1 >= (bz ? -1 : 0) && cz1 < nz;
+        // crossing fractions: reciprocal estimates, exact quotients only where
+        // the estimate cannot decide rs > rz (which picks the middle voxel)
+        const float dsv = sh - sl, dzv = zh - zl;
+        const float ns_ = (float)cs2 - sl, nz_ = (float)cz2 - zl;
+        float rs = bs ? ns_ * __builtin_amdgcn_rcpf(dsv) : 1.0f;
+        float rz = bz ? nz_ * __builtin_amdgcn_rcpf(dzv) : 1.0f;
+        const bool both = ok && bs && bz;
+        if (both && !(fabsf(rs - rz) > 4e-6f * fmaxf(rs, rz))) {
+            rs = ns_ / dsv;
+            rz = nz_ / dzv;
+        }
+        const bool sgt = rs > rz;
+        float wg = W;
+        if (TOF) {
+            const float dd = fmaf(pp1, R.tA, R.tB);
+            wg = W * __builtin_amdgcn_exp2f(-(dd * dd));
+        }
+        // the three candidate terms (flattened branch tree)
+        const float r1 = bs ? (bz ? (sgt ? rz : rs) : rs) : (bz ? rz : 1.0f);
+        const float r2 = sgt ? (rs - rz) : (rz - rs);
+        const float r3 = bs ? (bz ? (sgt ? 1.0f - rs : 1.0f - rz) : 1.0f - rs) : 1.0f - rz;
+        const int c2s = sgt ? cs1 : cs2, c2z = sgt ? cz2 : cz1;
+        const bool v1 = ok && cs1 >= 0 && cz1 >= 0;
+        const bool v2 = both & (sgt ? ((cs1 >= 0) & (cz2 < nz)) : ((cs2 < ns) & (cz1 >= 0)));
+        const bool v3 = ok && (bs || bz) && cs2 < ns && cz2 < nz;
+        const int a1 = cs1 - t.s0, b1 = cz1 - t.z0;
+        const int a2 = c2s - t.s0, b2 = c2z - t.z0;
+        const int a3 = cs2 - t.s0, b3 = cz2 - t.z0;
+        const int TS = xdom ? TY : TX;
+        const bool in1 = v1 && (unsigned)a1 < (unsigned)TS && (unsigned)b1 < (unsigned)TZ;
+        const bool in2 = v2 && (unsigned)a2 < (unsigned)TS && (unsigned)b2 < (unsigned)TZ;
+        const bool in3 = v3 && (unsigned)a3 < (unsigned)TS && (unsigned)b3 < (unsigned)TZ;
+        const int i1 = in1 ? base + a1 * t.ss + b1 * (TX * TY) : t.dummy;
+        const int i2 = in2 ? base + a2 * t.ss + b2 * (TX * TY) : t.dummy;
+        const int i3 = in3 ? base + a3 * t.ss + b3 * (TX * TY) : t.dummy;
+        if (MODE == M_GATHER) {
+            const float g1 = t.lds[i1], g2 = t.lds[i2], g3 = t.lds[i3];
+            t.acc += g1 * (r1 * wg);
+            t.acc += g2 * (r2 * wg);
+            t.acc += g3 * (r3 * wg);
+        } else if (MODE == M_SCATTER) {
+            // predicated: an LDS atomic costs a read-modify-write even for a dummy word
+            const float vw = t.value * wg;
+            if (in1) atomicAdd(t.lds + i1, vw * r1);
+            if (in2) atomicAdd(t.lds + i2, vw * r2);
+            if (in3) atomicAdd(t.lds + i3, vw * r3);
+        } else {
+            const int pg = ip * t.gsp;
+            const int bb1 = bs ? (bz ? (sgt ? 5 : 8) : 3) : (bz ? 1 : 0);
+            const int bb3 = bs ? (bz ? (sgt ? 7 : 10) : 4) : 2;
+            if (in1) {
+                t.h += mix64(((uint64_t)(uint32_t)(pg + cs1 * t.gss + cz1 * t.gsz) << 4) | (uint64_t)bb1);
+                t.cnt++;
+            }
+            if (in2) {
+                t.h += mix64(((uint64_t)(uint32_t)(pg + c2s * t.gss + c2z * t.gsz) << 4) | (uint64_t)(sgt ? 6 : 9));
+                t.cnt++;
+            }
+            if (in3) {
+                t.h += mix64(((uint64_t)(uint32_t)(pg + cs2 * t.gss + cz2 * t.gsz) << 4) | (uint64_t)bb3);
+                t.cnt++;
+            }
+        }
+        pp1 = pp2;
+        S0 = S1;
+        Z0 = Z1;
+        base += t.sp;
     }
-};
+}
 
 __device__ __forceinline__ void tile_origin(int tile, const TileCfg &T, int &x0, int &y0, int &z0)
 {
@@ -293,14 +396,21 @@ __device__ __forceinline__ void tile_origin(int tile, const TileCfg &T, int &x0,
     z0 = (tile / (T.ntx * T.nty)) * TZ;
 }
 
-template <class S>
-__device__ __forceinline__ void sink_for(S &s, float *lds, bool xdom, int x0, int y0, int z0)
+__device__ __forceinline__ void walk_for(TileWalk &t, float *lds, int dummy, bool xdom, int x0, int y0, int z0)
 {
-    s.lds = lds;
-    s.z0 = z0;
-    if (xdom) { s.p0 = x0; s.s0 = y0; s.sp = 1; s.ss = TX; }
-    else { s.p0 = y0; s.s0 = x0; s.sp = TX; s.ss = 1; }
+    t.lds = lds;
+    t.dummy = dummy;
+    t.z0 = z0;
+    if (xdom) { t.p0 = x0; t.s0 = y0; t.sp = 1; t.ss = TX; }
+    else { t.p0 = y0; t.s0 = x0; t.sp = TX; t.ss = 1; }
+    t.acc = 0.0f;
+    t.value = 0.0f;
+    t.h = 0;
+    t.cnt = 0;
 }
+
+// LDS: the tile plus one zero word (gathers) / one scratch word per lane (scatters).
+constexpr int LDS_WORDS = TVOX + TILE_THREADS;
 
 template <bool TOF, bool DS, bool DZ>
 __global__ __launch_bounds__(TILE_THREADS) void k_fp_tiled(const SegRec *__restrict__ recs,
@@ -308,7 +418,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fp_tiled(const SegRec *__restr
                                                            const float *__restrict__ img, LaunchGeom G, TileCfg T,
                                                            float *__restrict__ partial)
 {
-    __shared__ float tile[TVOX];
+    __shared__ float tile[LDS_WORDS];
     const Item it = items[blockIdx.x];
     int x0, y0, z0;
     tile_origin(it.tile, T, x0, y0, z0);
@@ -316,17 +426,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fp_tiled(const SegRec *__restr
         const int gx = x0 + (i % TX), gy = y0 + (i / TX) % TY, gz = z0 + i / (TX * TY);
         tile[i] = (gx < G.nx && gy < G.ny && gz < G.nz) ? img[gx + G.nx * (gy + G.ny * gz)] : 0.0f;
     }
+    if (threadIdx.x == 0) tile[TVOX] = 0.0f;
     __syncthreads();
     for (int r = it.start + threadIdx.x; r < it.start + it.count; r += TILE_THREADS) {
         const SegRec R = recs[r];
-        Branch B;
-        rec_branch(R, G, B);
-        TileGather s;
-        sink_for(s, tile, B.xdom, x0, y0, z0);
-        s.acc = 0.0f;
-        const TofLine TL{R.inv_pd, R.ratio1, R.L};
-        march_slabs<TOF, DS, DZ>(B, R.z1, G, (int)(R.ab_fp & 0xffffu), (int)(R.ab_fp >> 16), R.Wfp, TL, s);
-        partial[R.slot] = s.acc;
+        TileWalk t;
+        walk_for(t, tile, TVOX, (R.meta & 1u) != 0, x0, y0, z0);
+        march_tile<TOF, DS, DZ, M_GATHER>(R, G, (int)(R.ab_fp & 0xffffu), (int)(R.ab_fp >> 16), R.Wfp, t);
+        partial[R.slot] = t.acc;
     }
 }
 
@@ -339,7 +446,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restr
                                                            int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
                                                            float *__restrict__ tile_out)
 {
-    __shared__ float tile[TVOX];
+    __shared__ float tile[LDS_WORDS];
     const Item it = items[blockIdx.x];
     int x0, y0, z0;
     tile_origin(it.tile, T, x0, y0, z0);
@@ -358,34 +465,15 @@ __global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restr
         } else if (values) {
             v = values[R.ev];
         }
-        Branch B;
-        rec_branch(R, G, B);
-        TileScatter s;
-        sink_for(s, tile, B.xdom, x0, y0, z0);
-        s.value = v;
-        const TofLine TL{R.inv_pd, R.ratio1, R.L};
-        march_slabs<TOF, DS, DZ>(B, R.z1, G, a, b, adjoint ? R.Wfp : R.Wbp, TL, s);
+        TileWalk t;
+        walk_for(t, tile, TVOX + threadIdx.x, (R.meta & 1u) != 0, x0, y0, z0);
+        t.value = v;
+        march_tile<TOF, DS, DZ, M_SCATTER>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
     }
     __syncthreads();
     float *out = tile_out + (int64_t)blockIdx.x * TVOX;
     for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = tile[i];
 }
-
-struct TileDigest : TileSink {
-    int gsp, gss, gsz;       // global strides of primary / secondary / axial
-    uint64_t h;
-    int cnt;
-    __device__ __forceinline__ void term(int ip, int cs, int cz, int br, float)
-    {
-        bool in;
-        (void)slot(ip, cs, cz, in);
-        if (in) {
-            const uint32_t v = (uint32_t)(ip * gsp + cs * gss + cz * gsz);
-            h += mix64(((uint64_t)v << 4) | (uint64_t)br);
-            cnt += 1;
-        }
-    }
-};
 
 template <bool TOF, bool DS, bool DZ>
 __global__ __launch_bounds__(TILE_THREADS) void k_digest_tiled(const SegRec *__restrict__ recs,
@@ -400,20 +488,16 @@ __global__ __launch_bounds__(TILE_THREADS) void k_digest_tiled(const SegRec *__r
     for (int r = it.start + threadIdx.x; r < it.start + it.count; r += TILE_THREADS) {
         const SegRec R = recs[r];
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
-        Branch B;
-        rec_branch(R, G, B);
-        TileDigest d;
-        sink_for(d, nullptr, B.xdom, x0, y0, z0);
-        d.gsp = B.xdom ? 1 : G.nx;
-        d.gss = B.xdom ? G.nx : 1;
-        d.gsz = G.nx * G.ny;
-        d.h = 0;
-        d.cnt = 0;
-        const TofLine TL{R.inv_pd, R.ratio1, R.L};
-        march_slabs<TOF, DS, DZ>(B, R.z1, G, (int)(ab & 0xffffu), (int)(ab >> 16), 0.0f, TL, d);
-        if (d.cnt) {
-            atomicAdd(hash + R.ev, (unsigned long long)d.h);
-            atomicAdd(count + R.ev, d.cnt);
+        const bool xdom = (R.meta & 1u) != 0;
+        TileWalk t;
+        walk_for(t, nullptr, 0, xdom, x0, y0, z0);
+        t.gsp = xdom ? 1 : G.nx;
+        t.gss = xdom ? G.nx : 1;
+        t.gsz = G.nx * G.ny;
+        march_tile<TOF, DS, DZ, M_DIGEST>(R, G, (int)(ab & 0xffffu), (int)(ab >> 16), 0.0f, t);
+        if (t.cnt) {
+            atomicAdd(hash + R.ev, (unsigned long long)t.h);
+            atomicAdd(count + R.ev, t.cnt);
         }
     }
 }
