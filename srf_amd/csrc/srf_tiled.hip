@@ -264,7 +264,21 @@ __device__ __forceinline__ int cvt_flr(float v)
     return r;
 }
 
-enum : int { M_GATHER = 0, M_SCATTER = 1, M_DIGEST = 2 };
+enum : int { M_GATHER = 0, M_SCATTER = 1, M_DIGEST = 2, M_SCATTER_FX = 3 };
+
+// c * 2^shift as a 64-bit two's complement integer, exactly split into 32-bit
+// halves in float arithmetic (|c * 2^shift| < 2^41 by the choice of shift):
+// hi = floor(cs / 2^32) is an integer-valued float, lo = cs - hi * 2^32 lies in
+// [0, 2^32) and is exact (it keeps cs's 24-bit mantissa); the fraction below one
+// unit (2^-shift, at most 2^-40 of the largest term) is truncated.
+__device__ __forceinline__ unsigned long long to_fixed(float c, int shift)
+{
+    const float cs = ldexpf(c, shift);
+    const float hi = floorf(cs * 2.3283064365386963e-10f);
+    const float lo = fmaf(-hi, 4294967296.0f, cs);
+    const unsigned long long uh = (unsigned long long)(unsigned)(int)hi;
+    return (uh << 32) | (unsigned long long)(unsigned)lo;
+}
 
 // State of one segment walk inside one LDS tile (stored x fastest:
 // lds[lx + TX*(ly + TY*lz)]).  Terms whose cell lies outside the tile, or that
@@ -273,6 +287,8 @@ enum : int { M_GATHER = 0, M_SCATTER = 1, M_DIGEST = 2 };
 // branches and its three LDS accesses issue back to back.
 struct TileWalk {
     float *lds;
+    unsigned long long *fx;  // fixed-point accumulator tile
+    int shift;
     int dummy;
     int p0, s0, z0;    // tile origin along primary, secondary, axial
     int sp, ss;        // LDS strides of primary / secondary
@@ -360,11 +376,17 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
             t.acc += g2 * (r2 * wg);
             t.acc += g3 * (r3 * wg);
         } else if (MODE == M_SCATTER) {
-            // predicated: an LDS atomic costs a read-modify-write even for a dummy word
+            // float LDS atomics (non-finite values only; see k_bp_tiled)
             const float vw = t.value * wg;
             if (in1) atomicAdd(t.lds + i1, vw * r1);
             if (in2) atomicAdd(t.lds + i2, vw * r2);
             if (in3) atomicAdd(t.lds + i3, vw * r3);
+        } else if (MODE == M_SCATTER_FX) {
+            // 64-bit fixed point: integer LDS atomics run at full rate
+            const float vw = t.value * wg;
+            if (in1) atomicAdd(t.fx + i1, to_fixed(vw * r1, t.shift));
+            if (in2) atomicAdd(t.fx + i2, to_fixed(vw * r2, t.shift));
+            if (in3) atomicAdd(t.fx + i3, to_fixed(vw * r3, t.shift));
         } else {
             const int pg = ip * t.gsp;
             const int bb1 = bs ? (bz ? (sgt ? 5 : 8) : 3) : (bz ? 1 : 0);
@@ -437,7 +459,33 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fp_tiled(const SegRec *__restr
     }
 }
 
-// value source: values[e] (values != null), ratio(y[e], p[e]) (y != null), else 1
+__device__ __forceinline__ float event_value(const float *__restrict__ values, const float *__restrict__ y,
+                                             const float *__restrict__ p, int kind, float eps, int e)
+{
+    if (y) {
+        const float d = p[e] + eps;
+        float v = y[e] / d;
+        if (kind == SRF_RATIO_DIV_NO_NAN && d == 0.0f) v = 0.0f;
+        return v;
+    }
+    return values ? values[e] : 1.0f;
+}
+
+// Backprojection of one work item into its LDS tile, written out once as float.
+//
+// Terms are accumulated in 64-bit fixed point: integer LDS atomics run at full
+// rate where float LDS atomics are serialised per lane, and integer addition
+// is associative, so the result is bit-reproducible.  The scale is chosen per
+// item: a pre-pass keeps each lane's event values in registers and finds the
+// item's largest |value * W| (W bounds every term weight of a segment), so
+// every term is below 2^49.  A segment puts at most one term on a voxel and an
+// item has at most ITEM = 2^13 segments, so no voxel sum can reach 2^62.
+// Items holding a non-finite value use float LDS atomics instead (NaN and inf
+// then propagate as through the reference's atomicAdd).
+constexpr int FX_BITS = 49;
+constexpr int RECS_PER_THREAD = ITEM / TILE_THREADS;
+static_assert(ITEM % TILE_THREADS == 0, "whole records per thread");
+
 template <bool TOF, bool DS, bool DZ>
 __global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restrict__ recs,
                                                            const Item *__restrict__ items,
@@ -446,33 +494,71 @@ __global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restr
                                                            int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
                                                            float *__restrict__ tile_out)
 {
-    __shared__ float tile[LDS_WORDS];
+    __shared__ unsigned long long fx[TVOX];
+    __shared__ unsigned s_cmax;
+    __shared__ int s_bad;
+    float *fl = reinterpret_cast<float *>(fx);
     const Item it = items[blockIdx.x];
     int x0, y0, z0;
     tile_origin(it.tile, T, x0, y0, z0);
-    for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) tile[i] = 0.0f;
+    if (threadIdx.x == 0) {
+        s_cmax = 0u;
+        s_bad = 0;
+    }
+    for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) fx[i] = 0ull;
+    const int end = it.start + it.count;
+    float vv[RECS_PER_THREAD];
+    unsigned m = 0u;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < RECS_PER_THREAD; ++k) {
+        const int r = it.start + threadIdx.x + k * TILE_THREADS;
+        vv[k] = 0.0f;
+        if (r < end) {
+            const uint32_t ab = adjoint ? recs[r].ab_fp : recs[r].ab_bp;
+            if ((ab & 0xffffu) < (ab >> 16)) {
+                const float v = event_value(values, y, p, kind, eps, recs[r].ev);
+                vv[k] = v;
+                const float c = fabsf(v) * (adjoint ? recs[r].Wfp : recs[r].Wbp);
+                if (isfinite(c)) m = max(m, __float_as_uint(c));
+                else bad = true;
+            }
+        }
+    }
     __syncthreads();
-    for (int r = it.start + threadIdx.x; r < it.start + it.count; r += TILE_THREADS) {
+    if (m) atomicMax(&s_cmax, m);
+    if (bad) s_bad = 1;
+    __syncthreads();
+    const bool fixed = s_bad == 0;
+    int shift = 0;
+    if (fixed && s_cmax) {
+        int ex = 0;
+        (void)frexpf(__uint_as_float(s_cmax), &ex);   // cmax < 2^ex
+        shift = FX_BITS - ex;
+    }
+#pragma unroll
+    for (int k = 0; k < RECS_PER_THREAD; ++k) {
+        const int r = it.start + threadIdx.x + k * TILE_THREADS;
+        if (r >= end) continue;
         const SegRec R = recs[r];
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
         const int a = (int)(ab & 0xffffu), b = (int)(ab >> 16);
         if (a >= b) continue;
-        float v = 1.0f;
-        if (y) {
-            const float d = p[R.ev] + eps;
-            v = y[R.ev] / d;
-            if (kind == SRF_RATIO_DIV_NO_NAN && d == 0.0f) v = 0.0f;
-        } else if (values) {
-            v = values[R.ev];
-        }
         TileWalk t;
-        walk_for(t, tile, TVOX + threadIdx.x, (R.meta & 1u) != 0, x0, y0, z0);
-        t.value = v;
-        march_tile<TOF, DS, DZ, M_SCATTER>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
+        walk_for(t, fl, TVOX, (R.meta & 1u) != 0, x0, y0, z0);
+        t.fx = fx;
+        t.shift = shift;
+        t.value = vv[k];
+        if (fixed) march_tile<TOF, DS, DZ, M_SCATTER_FX>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
+        else march_tile<TOF, DS, DZ, M_SCATTER>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
     }
     __syncthreads();
     float *out = tile_out + (int64_t)blockIdx.x * TVOX;
-    for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = tile[i];
+    if (fixed) {
+        for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = ldexpf((float)(long long)fx[i], -shift);
+    } else {
+        for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = fl[i];
+    }
 }
 
 template <bool TOF, bool DS, bool DZ>

@@ -109,3 +109,24 @@ def test_plan_empty_and_nan():
     bp = plan.backproject(None, 0)
     assert float(bp.abs().sum()) == 0.0
     plan.close()
+
+
+def test_plan_backprojection_is_deterministic_and_propagates_nan():
+    """Fixed-point LDS accumulation: two runs are bit-identical; a NaN value
+    switches to the float path and poisons the voxels its event touches."""
+    import torch
+    from srf_amd import ops
+    from siddon_ref import random_cylinder_lors
+    lors = torch.from_numpy(random_cylinder_lors(100000, 150.0, 80.0, 9, tof_sigma=25.0)).cuda()
+    shape, geom = (48, 48, 24), [0.0, 0.0, 0.0, 96.0, 96.0, 48.0]
+    plan = ops.Plan(lors, shape, geom, True, 3.66, 649.2)
+    v = torch.rand(lors.shape[0], device="cuda") * 3.0 - 1.0     # signed values
+    b1 = plan.backproject(v, 0)
+    b2 = plan.backproject(v, 0)
+    assert torch.equal(b1, b2)
+    ref = ops.backproject(lors, v, list(shape), geom, True, 3.66, 649.2, 0)
+    assert rel_l2(b1.cpu().numpy(), ref.cpu().numpy()) <= 1e-5
+    v[123] = float("nan")
+    b3 = plan.backproject(v, 0)
+    assert torch.isnan(b3).any() and not torch.isnan(b3).all()
+    plan.close()
