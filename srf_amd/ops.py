@@ -378,5 +378,8 @@ def _(handle, y, p, kind, eps, n, shape, mode):
     return y.new_empty((nz, ny, nx))
 
 
+# gradients of srf::project / srf::plan_project (adjoint-mode backprojector)
+from . import autograd as _autograd  # noqa: E402,F401
+
 __all__ = ["Plan", "plan_project", "plan_backproject", "plan_backproject_ratio", "project", "backproject", "backproject_ratio", "ratio", "em_update", "lor_digest",
            "to_kernel_layout", "from_kernel_layout", "_lib"]
