@@ -133,13 +133,13 @@ def main():
             a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
             a.record(stream)
         if plan is not None:
-            p = plan.project(eng.x)
+            q = plan.project_ratio(eng.x, y_s, 1, 1e-5)
         else:
             p = ops.project(lors_s, eng.x, eng.shape, eng.geom, True, tof_bin, tof_sigma2)
         if timed:
             b.record(stream)
         if plan is not None:
-            bp = plan.backproject_ratio(y_s, p, 1, 1e-5, 0)
+            bp = plan.backproject(q, 0)
         else:
             bp = ops.backproject_ratio(lors_s, y_s, p, 1, 1e-5, eng.shape, eng.geom, True, tof_bin, tof_sigma2, 0)
         if timed:

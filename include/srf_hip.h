@@ -152,6 +152,12 @@ int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments,
                   int64_t *n_items, size_t *device_bytes);
 /* out[e] = projection of event e (srf_project semantics). */
 int srf_plan_project(srf_plan *plan, const float *image, float *out, void *stream);
+/* q[e] = ratio(y[e], p[e]) with p = the projection (srf_plan_project then
+ * srf_ratio, fused: the ratio is applied in the per-event reduction).  The EM
+ * sub-iteration's forward half (mlem.py:40-42, mlem_full_tof.py:108-109,
+ * osem_full.py:218-220). */
+int srf_plan_project_ratio(srf_plan *plan, const float *image, const float *y, srf_ratio_kind_t kind,
+                           float eps, float *q, void *stream);
 /* image (+)= backprojection (srf_backproject semantics; values NULL = ones). */
 int srf_plan_backproject(srf_plan *plan, const float *values, srf_bp_mode_t mode,
                          int accumulate, float *image, void *stream);

@@ -63,6 +63,7 @@ SIGNATURES = [
     ("srf_plan_backproject_ratio", ctypes.c_int, [_P, _P, _P, ctypes.c_int, _F, ctypes.c_int, ctypes.c_int, _P,
                                                   _P]),
     ("srf_plan_digest", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P]),
+    ("srf_plan_project_ratio", ctypes.c_int, [_P, _P, _P, ctypes.c_int, _F, _P, _P]),
 ]
 
 _lib = None

@@ -64,6 +64,8 @@ static_assert(sizeof(SegRec) == 64, "record is one 64-byte line");
 
 struct Item {
     int start, count, tile, pad;
+    float wmax_bp, wmax_fp;   // largest weight constant of the item's records, per mode
+    float pad2[2];
 };
 
 // ------------------------------------------------------------ enumeration --
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
                                                           const float *__restrict__ tofv, int tof_stride,
                                                           LaunchGeom G, TileCfg T, const int *__restrict__ ev_off,
                                                           unsigned *__restrict__ key_cursor,
-                                                          SegRec *__restrict__ recs)
+                                                          SegRec *__restrict__ recs, int *__restrict__ rec_ev)
 {
     extern __shared__ unsigned hist[];
     for (int i = threadIdx.x; i < T.nkeys; i += BIN_THREADS) hist[i] = 0;
@@ -249,6 +251,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
             R.meta = B.xdom ? 1u : 0u;
             R.pad[0] = R.pad[1] = 0u;
             recs[pos] = R;
+            rec_ev[pos] = (int)e;
         });
     }
 }
@@ -476,9 +479,10 @@ __device__ __forceinline__ float event_value(const float *__restrict__ values, c
 // Terms are accumulated in 64-bit fixed point: integer LDS atomics run at full
 // rate where float LDS atomics are serialised per lane, and integer addition
 // is associative, so the result is bit-reproducible.  The scale is chosen per
-// item: a pre-pass keeps each lane's event values in registers and finds the
-// item's largest |value * W| (W bounds every term weight of a segment), so
-// every term is below 2^49.  A segment puts at most one term on a voxel and an
+// item: a pre-pass (event ids streamed from rec_ev) keeps each lane's event
+// values in registers, and the item's largest |value| times its largest weight
+// constant W (precomputed per mode; W bounds every term weight of a segment)
+// bounds every term, which is scaled below 2^49.  A segment puts at most one term on a voxel and an
 // item has at most ITEM = 2^13 segments, so no voxel sum can reach 2^62.
 // Items holding a non-finite value use float LDS atomics instead (NaN and inf
 // then propagate as through the reference's atomicAdd).
@@ -492,6 +496,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restr
                                                            const float *__restrict__ values,
                                                            const float *__restrict__ y, const float *__restrict__ p,
                                                            int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
+                                                           const int *__restrict__ rec_ev,
                                                            float *__restrict__ tile_out)
 {
     __shared__ unsigned long long fx[TVOX];
@@ -508,23 +513,21 @@ __global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restr
     for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) fx[i] = 0ull;
     const int end = it.start + it.count;
     float vv[RECS_PER_THREAD];
-    unsigned m = 0u;
+    float vmax = 0.0f;
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < RECS_PER_THREAD; ++k) {
         const int r = it.start + threadIdx.x + k * TILE_THREADS;
         vv[k] = 0.0f;
         if (r < end) {
-            const uint32_t ab = adjoint ? recs[r].ab_fp : recs[r].ab_bp;
-            if ((ab & 0xffffu) < (ab >> 16)) {
-                const float v = event_value(values, y, p, kind, eps, recs[r].ev);
-                vv[k] = v;
-                const float c = fabsf(v) * (adjoint ? recs[r].Wfp : recs[r].Wbp);
-                if (isfinite(c)) m = max(m, __float_as_uint(c));
-                else bad = true;
-            }
+            const float v = event_value(values, y, p, kind, eps, rec_ev[r]);
+            vv[k] = v;
+            if (isfinite(v)) vmax = fmaxf(vmax, fabsf(v));
+            else bad = true;
         }
     }
+    const float c = vmax * (adjoint ? it.wmax_fp : it.wmax_bp);
+    const unsigned m = isfinite(c) ? __float_as_uint(c) : 0x7f7fffffu;
     __syncthreads();
     if (m) atomicMax(&s_cmax, m);
     if (bad) s_bad = 1;
@@ -606,16 +609,51 @@ __global__ __launch_bounds__(256) void k_tile_reduce(const float *__restrict__ t
     img[v] = accumulate ? img[v] + s : s;
 }
 
+// Per item, the largest weight constant of its records in each mode.
+__global__ __launch_bounds__(256) void k_item_wmax(const SegRec *__restrict__ recs, Item *__restrict__ items)
+{
+    __shared__ float sb[256], sf[256];
+    const Item it = items[blockIdx.x];
+    float mb = 0.0f, mf = 0.0f;
+    for (int r = it.start + threadIdx.x; r < it.start + it.count; r += 256) {
+        mb = fmaxf(mb, fabsf(recs[r].Wbp));
+        mf = fmaxf(mf, fabsf(recs[r].Wfp));
+    }
+    sb[threadIdx.x] = mb;
+    sf[threadIdx.x] = mf;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            sb[threadIdx.x] = fmaxf(sb[threadIdx.x], sb[threadIdx.x + k]);
+            sf[threadIdx.x] = fmaxf(sf[threadIdx.x], sf[threadIdx.x + k]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        items[blockIdx.x].wmax_bp = sb[0];
+        items[blockIdx.x].wmax_fp = sf[0];
+    }
+}
+
 // p[e] = sum of e's segment partials (in segment order); NaN for non-finite LORs.
+// With y != null the EM ratio is applied on the way out: out[e] = ratio(y[e], p[e]).
 __global__ __launch_bounds__(256) void k_seg_reduce(const float *__restrict__ partial, const int *__restrict__ ev_off,
                                                     const uint8_t *__restrict__ ev_flag, int64_t n,
+                                                    const float *__restrict__ y, int kind, float eps,
                                                     float *__restrict__ out)
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
     float s = 0.0f;
     for (int k = ev_off[e]; k < ev_off[e + 1]; ++k) s += partial[k];
-    out[e] = ev_flag[e] ? __builtin_nanf("") : s;
+    if (ev_flag[e]) s = __builtin_nanf("");
+    if (y) {
+        const float d = s + eps;
+        float q = y[e] / d;
+        if (kind == SRF_RATIO_DIV_NO_NAN && d == 0.0f) q = 0.0f;
+        s = q;
+    }
+    out[e] = s;
 }
 
 __global__ void k_fill_u32(unsigned *__restrict__ d, unsigned v, int64_t n)
@@ -635,6 +673,7 @@ struct srf_plan {
     int64_t n, nseg;
     int nitems;
     SegRec *recs;
+    int *rec_ev;
     Item *items;
     int *tile_items;
     int *ev_off;
@@ -664,6 +703,7 @@ void plan_free(srf_plan *P)
 {
     if (!P) return;
     (void)hipFree(P->recs);
+    (void)hipFree(P->rec_ev);
     (void)hipFree(P->items);
     (void)hipFree(P->tile_items);
     (void)hipFree(P->ev_off);
@@ -775,7 +815,7 @@ int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const floa
         const int tile = k / NCLASS;
         if (k % NCLASS == 0) tile_items[tile] = (int)items.size();
         for (unsigned st = 0; st < kc[k]; st += ITEM)
-            items.push_back(Item{(int)(acc + st), (int)std::min<unsigned>(ITEM, kc[k] - st), tile, 0});
+            items.push_back(Item{(int)(acc + st), (int)std::min<unsigned>(ITEM, kc[k] - st), tile, 0, 0.0f, 0.0f, {0.0f, 0.0f}});
         acc += kc[k];
     }
     tile_items[T.ntiles] = (int)items.size();
@@ -785,6 +825,7 @@ int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const floa
     }
     P->nitems = (int)items.size();
     SRF_TRY(dmalloc(&P->recs, (size_t)P->nseg, P->bytes));
+    SRF_TRY(dmalloc(&P->rec_ev, (size_t)P->nseg, P->bytes));
     SRF_TRY(dmalloc(&P->partial, (size_t)P->nseg, P->bytes));
     SRF_TRY(dmalloc(&P->items, items.size(), P->bytes));
     SRF_TRY(dmalloc(&P->tile_items, tile_items.size(), P->bytes));
@@ -797,11 +838,12 @@ int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const floa
     if (n > 0) {
         if (P->tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs);
+                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs, P->rec_ev);
         else
             hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs);
+                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs, P->rec_ev);
     }
+    if (P->nitems > 0) hipLaunchKernelGGL(k_item_wmax, dim3((unsigned)P->nitems), dim3(256), 0, s, P->recs, P->items);
     SRF_HIP(hipGetLastError());
     SRF_HIP(hipStreamSynchronize(s));
     (void)hipFree(ev_cnt);
@@ -850,7 +892,23 @@ int srf_plan_project(srf_plan *P, const float *image, float *out, void *stream)
         PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, image, P->G, P->T,
                                                                        P->partial));
     hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial, P->ev_off,
-                       P->ev_flag, P->n, out);
+                       P->ev_flag, P->n, (const float *)nullptr, 0, 0.0f, out);
+    return launch_status();
+}
+
+int srf_plan_project_ratio(srf_plan *P, const float *image, const float *y, srf_ratio_kind_t kind, float eps,
+                           float *q, void *stream)
+{
+    if (!P) return SRF_ERR_INVALID_ARG;
+    if (kind != SRF_RATIO_PLAIN && kind != SRF_RATIO_DIV_NO_NAN) return SRF_ERR_INVALID_ARG;
+    if (P->n == 0) return SRF_OK;
+    if (!image || !y || !q) return SRF_ERR_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (P->nitems > 0)
+        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, image, P->G, P->T,
+                                                                       P->partial));
+    hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial, P->ev_off,
+                       P->ev_flag, P->n, y, (int)kind, eps, q);
     return launch_status();
 }
 
@@ -862,7 +920,7 @@ static int plan_bp(srf_plan *P, const float *values, const float *y, const float
     if (P->nitems > 0)
         PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, values, y, p, kind, eps,
                                                                        (int)(mode == SRF_BP_ADJOINT), P->G, P->T,
-                                                                       P->tile_buf));
+                                                                       P->rec_ev, P->tile_buf));
     dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
     hipLaunchKernelGGL(k_tile_reduce, grid, dim3(256), 0, s, P->tile_buf, P->tile_items, P->G, P->T, accumulate,
                        image);

@@ -168,8 +168,8 @@ class EmEngine:
             plan = self.plan
         y = self.y if y is None else y
         if plan is not None:
-            p = plan.project(self.x)
-            bp = plan.backproject_ratio(y, p, self.ratio_kind, self.eps, 0)
+            q = plan.project_ratio(self.x, y, self.ratio_kind, self.eps)
+            bp = plan.backproject(q, 0)
         else:
             lors = self.lors if lors is None else lors
             p = ops.project(lors, self.x, self.shape, self.geom, self.use_tof, self.tof_bin, self.tof_sigma2)

@@ -289,6 +289,10 @@ class Plan:
     def project(self, image_k: Tensor) -> Tensor:
         return plan_project(self.handle, image_k, self.n, self.shape)
 
+    def project_ratio(self, image_k: Tensor, y: Tensor, kind: int, eps: float) -> Tensor:
+        """q = ratio(y, project(image)) in one pass (the EM forward half)."""
+        return plan_project_ratio(self.handle, image_k, y, int(kind), float(eps), self.n, self.shape)
+
     def backproject(self, values: Optional[Tensor] = None, mode: int = 0) -> Tensor:
         return plan_backproject(self.handle, values, self.n, self.shape, int(mode))
 
@@ -330,6 +334,27 @@ def plan_project(handle: int, image_k: Tensor, n: int, shape: List[int]) -> Tens
 
 @plan_project.register_fake
 def _(handle, image_k, n, shape):
+    return image_k.new_empty(n)
+
+
+@torch.library.custom_op("srf::plan_project_ratio", mutates_args=())
+def plan_project_ratio(handle: int, image_k: Tensor, y: Tensor, kind: int, eps: float, n: int,
+                       shape: List[int]) -> Tensor:
+    """q[e] = ratio(y[e], p[e]), p = projection of every event of a plan."""
+    _check_image(image_k, shape)
+    _req(y, "y")
+    if y.numel() != n:
+        raise ValueError("y must have one entry per event")
+    out = torch.empty(n, dtype=torch.float32, device=image_k.device)
+    with torch.cuda.device(image_k.device):
+        rc = lib().srf_plan_project_ratio(handle, image_k.data_ptr(), y.data_ptr(), int(kind), float(eps),
+                                          out.data_ptr(), stream_handle(image_k.device))
+    check(rc, "srf_plan_project_ratio")
+    return out
+
+
+@plan_project_ratio.register_fake
+def _(handle, image_k, y, kind, eps, n, shape):
     return image_k.new_empty(n)
 
 

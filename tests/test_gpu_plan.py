@@ -130,3 +130,20 @@ def test_plan_backprojection_is_deterministic_and_propagates_nan():
     b3 = plan.backproject(v, 0)
     assert torch.isnan(b3).any() and not torch.isnan(b3).all()
     plan.close()
+
+
+def test_plan_project_ratio_fused():
+    """srf_plan_project_ratio == srf_ratio(y, srf_plan_project) bit for bit."""
+    import torch
+    from srf_amd import ops
+    from siddon_ref import random_cylinder_lors
+    lors = torch.from_numpy(random_cylinder_lors(50000, 150.0, 80.0, 19, tof_sigma=25.0)).cuda()
+    shape, geom = (48, 48, 24), [0.0, 0.0, 0.0, 96.0, 96.0, 48.0]
+    plan = ops.Plan(lors, shape, geom, True, 3.66, 649.2)
+    img = torch.rand((24, 48, 48), device="cuda")
+    y = torch.rand(lors.shape[0], device="cuda")
+    p = plan.project(img)
+    for kind, eps in ((0, 1e-8), (1, 1e-5), (1, 0.0)):
+        q = plan.project_ratio(img, y, kind, eps)
+        assert torch.equal(q, ops.ratio(y, p, kind, eps))
+    plan.close()
