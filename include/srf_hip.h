@@ -170,6 +170,25 @@ int srf_plan_backproject_ratio(srf_plan *plan, const float *y, const float *p,
 int srf_plan_digest(srf_plan *plan, srf_bp_mode_t mode, uint64_t *hash, int32_t *count,
                     void *stream);
 
+/* ----------------------------------------------------- scatter estimate --
+ * Single-scatter simulation on a virtual-detector sinogram: the per-LOR
+ * Klein-Nishina sum of srfnef/corrections/scattering/scatter_fraction/
+ * scatter_fraction.py:12-136 (loop_all_lors / loop_all_s) with the line
+ * integrals of scatter/preprocess.py:21-43 supplied by the projector.
+ *   det     (n_det, 4)   detector x, y, z (mm), normal angle (rad)
+ *   points  (n_points,4) scatter point x, y, z (mm), mu at the point (1/mm)
+ *   em_int, mu_int (n_det, n_points) emission / attenuation line integrals
+ *   rows    workspace of srf_sss_rows_bytes() bytes (16-byte aligned)
+ *   bins    (n_bins, 2)  detector pairs; out (n_bins)
+ *   tables  543 floats: 511/E', window efficiency, KN/sigma_KN per integer
+ *           scattering angle 0..180 degrees. */
+size_t srf_sss_rows_bytes(int32_t n_det, int32_t n_points);
+int srf_sss_prepare(const float *det, int32_t n_det, const float *points, int32_t n_points,
+                    const float *em_int, const float *mu_int, void *rows, size_t rows_bytes, void *stream);
+int srf_sss_sinogram(const void *rows, int32_t n_det, const float *points, int32_t n_points,
+                     const int32_t *bins, int32_t n_bins, const float *tables, float scale, float *out,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,9 @@ SIGNATURES = [
                                                   _P]),
     ("srf_plan_digest", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P]),
     ("srf_plan_project_ratio", ctypes.c_int, [_P, _P, _P, ctypes.c_int, _F, _P, _P]),
+    ("srf_sss_rows_bytes", ctypes.c_size_t, [_I32, _I32]),
+    ("srf_sss_prepare", ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _P, ctypes.c_size_t, _P]),
+    ("srf_sss_sinogram", ctypes.c_int, [_P, _I32, _P, _I32, _P, _I32, _P, _F, _P, _P]),
 ]
 
 _lib = None

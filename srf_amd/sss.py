@@ -1,0 +1,255 @@
+"""Single-scatter simulation (SSS) and scatter correction.
+
+The physics is the reference's (srfnef/corrections/scattering/):
+  * scatter points on a regular grid of the emission image
+    (preprocess.py:6-18 pre_all_scatter_position: step = floor(spacing/voxel));
+  * emission and attenuation line integrals from every scatter point to every
+    detector, computed by the same projector that reconstructs
+    (preprocess.py:21-43 uses Project / UmapProject for them);
+  * per LOR a Klein-Nishina weighted sum over scatter points
+    (scatter_fraction/scatter_fraction.py:40-131, kn_formula.py), evaluated by
+    the HIP kernel srf_sss_sinogram;
+  * correction by subtracting the normalised scatter estimate scaled to a
+    fixed scatter fraction of the counts (scatter.py:51-53).
+
+Where the reference runs over every crystal pair of a block scanner, this
+evaluates a coarse virtual-detector sinogram: n_det detectors per ring on
+n_rings rings of the scanner cylinder, direct planes only, n_angles x
+n_radial bins per ring; bins snap to the nearest detectors.  Attenuation uses
+the physical line integral of mu (the reference's UmapProject scaling
+L^2/unit_size[0] is exact only for 1 mm voxels, quirk Q15).
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import ops
+from ._lib import check, lib, stream_handle
+from .data import Listmode, nef_class
+
+SIGMA_KN = math.pi * (40.0 / 9.0 - 3.0 * math.log(3.0))
+
+
+def energy_tables(low, high, resolution, physical_kn=False):
+    """(543,) float32: per integer scattering angle 0..180 deg: 511/E',
+    window efficiency eff(E'), KN(c)/sigma_KN with c the quantised angle used
+    as a cosine (kn_formula.py:12,16; quirk Q16) or cos of it (physical_kn)."""
+    k = np.arange(181, dtype=np.float64)
+    th = k / 180.0 * math.pi
+    c = np.cos(th)
+    Ep = 511.0 / (2.0 - c)
+    cc = c if physical_kn else th
+    kn = 1.0 / (2.0 - cc) ** 2 * ((1.0 + cc ** 2) / 2.0 + (1.0 - cc) ** 2 / 2.0 / (2.0 - cc)) / SIGMA_KN
+    s = math.sqrt(511.0) * resolution
+    Ei = np.arange(int(low), int(high), 5, dtype=np.float64)
+    eff = np.sum(np.exp(-(Ei[None, :] - Ep[:, None]) ** 2 / 2.0 / s ** 2), axis=1)
+    return np.concatenate([511.0 / Ep, eff, kn]).astype(np.float32)
+
+
+def sss_scale(low, high, resolution):
+    """5/sqrt(1022 pi)/resolution * eff(511) / (4 pi) (scatter_fraction.py:27-33)."""
+    s = math.sqrt(511.0) * resolution
+    Ei = np.arange(int(low), int(high), 5, dtype=np.float64)
+    e511 = float(np.sum(np.exp(-(Ei - 511.0) ** 2 / 2.0 / s ** 2)))
+    return 5.0 / math.sqrt(1022.0 * math.pi) / resolution * e511 / (4 * math.pi)
+
+
+@nef_class
+class SssGeometry:
+    """Virtual detectors and the coarse sinogram (defaults: 424 mm radius,
+    300 mm axial, 128 detectors x 32 rings, 64 angles x 64 radial bins)."""
+    radius: float = 424.0
+    axial_length: float = 300.0
+    n_det: int = 128
+    n_rings: int = 32
+    n_angles: int = 64
+    n_radial: int = 64
+    fov_radius: float = 256.0
+    spacing: float = 8.0
+    low_energy: float = 400.0
+    high_energy: float = 650.0
+    resolution: float = 0.15
+
+    def detectors(self, device):
+        k = torch.arange(self.n_det, dtype=torch.float64)
+        r = torch.arange(self.n_rings, dtype=torch.float64)
+        phi = 2 * math.pi * k / self.n_det
+        z = (r + 0.5) * self.axial_length / self.n_rings - self.axial_length / 2
+        det = torch.zeros((self.n_rings, self.n_det, 4), dtype=torch.float64)
+        det[:, :, 0] = self.radius * torch.cos(phi)[None, :]
+        det[:, :, 1] = self.radius * torch.sin(phi)[None, :]
+        det[:, :, 2] = z[:, None]
+        det[:, :, 3] = phi[None, :]
+        return det.reshape(-1, 4).to(torch.float32).to(device)
+
+    def bin_detectors(self):
+        """(n_rings*n_angles*n_radial, 2) detector pairs of the sinogram bins
+        (ring-major, then angle, then radial)."""
+        a = (np.arange(self.n_angles) + 0.0) * math.pi / self.n_angles
+        ds = 2.0 * self.fov_radius / self.n_radial
+        s = (np.arange(self.n_radial) + 0.5 - self.n_radial / 2) * ds
+        A, S = np.meshgrid(a, s, indexing="ij")
+        t = np.sqrt(np.maximum(self.radius ** 2 - S ** 2, 0.0))
+        nx, ny = np.cos(A), np.sin(A)
+        dx, dy = -ny, nx
+        psi1 = np.arctan2(S * ny - t * dy, S * nx - t * dx)
+        psi2 = np.arctan2(S * ny + t * dy, S * nx + t * dx)
+        k1 = np.mod(np.rint(psi1 / (2 * math.pi) * self.n_det), self.n_det).astype(np.int64)
+        k2 = np.mod(np.rint(psi2 / (2 * math.pi) * self.n_det), self.n_det).astype(np.int64)
+        rings = np.arange(self.n_rings)[:, None, None] * self.n_det
+        b = np.stack([rings + k1[None], rings + k2[None]], -1).reshape(-1, 2)
+        return b.astype(np.int32)
+
+    def bin_of_lors(self, lors):
+        """Sinogram bin of each LOR (N,>=6) tensor: transaxial angle and signed
+        distance of the line, ring of the LOR midpoint (single-slice rebinning)."""
+        p1, p2 = lors[:, 0:3].double(), lors[:, 3:6].double()
+        d = p2[:, :2] - p1[:, :2]
+        n = torch.stack([-d[:, 1], d[:, 0]], 1)
+        n = n / torch.clamp(n.norm(dim=1, keepdim=True), min=1e-12)
+        phi = torch.atan2(n[:, 1], n[:, 0])
+        flip = (phi < 0) | (phi >= math.pi)
+        n = torch.where(flip[:, None], -n, n)
+        phi = torch.atan2(n[:, 1], n[:, 0]).clamp(0, math.pi - 1e-12)
+        s = (p1[:, :2] * n).sum(1)
+        a = torch.clamp((phi / math.pi * self.n_angles).long(), 0, self.n_angles - 1)
+        rho = torch.clamp(((s + self.fov_radius) / (2 * self.fov_radius) * self.n_radial).floor().long(), 0,
+                          self.n_radial - 1)
+        zm = 0.5 * (p1[:, 2] + p2[:, 2])
+        r = torch.clamp(((zm + self.axial_length / 2) / self.axial_length * self.n_rings).floor().long(), 0,
+                        self.n_rings - 1)
+        return (r * self.n_angles + a) * self.n_radial + rho
+
+    def scatter_points(self, mu_image, center, size, device):
+        """(ns, 4): grid points of preprocess.py:6-18 (step = floor(spacing /
+        voxel) voxels) with mu at the point's voxel; points outside the body
+        carry mu = 0 and drop out of the sum."""
+        shape = mu_image.shape
+        vox = [size[i] / shape[i] for i in range(3)]
+        step = [max(1, int(math.floor(self.spacing / vox[i]))) for i in range(3)]
+        cnt = [int(shape[i] // step[i]) for i in range(3)]
+        g = [torch.arange(cnt[i], dtype=torch.float64) for i in range(3)]
+        X, Y, Z = torch.meshgrid(*g, indexing="ij")
+        pos = torch.stack([step[0] * (X + 0.5) * vox[0] - size[0] / 2 + center[0],
+                           step[1] * (Y + 0.5) * vox[1] - size[1] / 2 + center[1],
+                           step[2] * (Z + 0.5) * vox[2] - size[2] / 2 + center[2]], -1).reshape(-1, 3)
+        ix = (step[0] * X + step[0] // 2).long().reshape(-1)
+        iy = (step[1] * Y + step[1] // 2).long().reshape(-1)
+        iz = (step[2] * Z + step[2] // 2).long().reshape(-1)
+        mu = torch.as_tensor(mu_image, device="cpu").to(torch.float64)[ix, iy, iz]
+        out = torch.cat([pos, mu[:, None]], 1).to(torch.float32)
+        return out.to(device)
+
+
+class SingleScatterSimulation:
+    """SSS sinogram of (emission, mu) images on a fixed geometry.
+
+    The (detector, scatter point) LOR set depends only on the geometry and the
+    image grid, so it is binned once into a projector plan and reused for
+    every (emission, mu) pair; scatter points follow the mu map of each call."""
+
+    def __init__(self, geometry, shape, center, size, physical_kn=False, tiled=False):
+        self.g = geometry
+        self.shape = [int(v) for v in shape]
+        self.center = [float(v) for v in center]
+        self.size = [float(v) for v in size]
+        self.geom = self.center + self.size
+        self.dev = torch.device("cuda", torch.cuda.current_device())
+        self.det = geometry.detectors(self.dev)
+        self.nd = self.det.shape[0]
+        # the full scatter grid (mu decides which points count)
+        grid = geometry.scatter_points(np.zeros(self.shape, np.float32), self.center, self.size, self.dev)
+        self.pts_xyz = grid[:, :3].contiguous()
+        self.ns = self.pts_xyz.shape[0]
+        d = self.det[:, None, :3].expand(self.nd, self.ns, 3)
+        sp = self.pts_xyz[None, :, :].expand(self.nd, self.ns, 3)
+        # p1 = detector, p2 = scatter point: the projector's segment cut ends the
+        # integral at the point (siddon.cu.cc:56-57; preprocess.py:46-54 order)
+        self.lors = torch.cat([d, sp], -1).reshape(-1, 6).contiguous()
+        L2 = ((self.lors[:, 3:6] - self.lors[:, 0:3]) ** 2).sum(1)
+        dx_ = self.size[0] / self.shape[0]
+        self.to_integral = (L2 / (dx_ * dx_)).contiguous()     # Project * L^2 / dx_^2 = integral
+        self.plan = ops.Plan(self.lors, self.shape, self.geom, False) if tiled else None
+        self.bins = torch.from_numpy(geometry.bin_detectors()).to(self.dev).contiguous()
+        self.tables = torch.from_numpy(energy_tables(geometry.low_energy, geometry.high_energy,
+                                                     geometry.resolution, physical_kn)).to(self.dev)
+        self.scale = sss_scale(geometry.low_energy, geometry.high_energy, geometry.resolution)
+        self.rows = torch.empty(int(lib().srf_sss_rows_bytes(self.nd, self.ns)) // 4 + 4, dtype=torch.float32,
+                                device=self.dev)
+
+    def line_integrals(self, img_xyz):
+        img_k = ops.to_kernel_layout(img_xyz.contiguous())
+        if self.plan is not None:
+            p = self.plan.project(img_k)
+        else:
+            p = ops.project(self.lors, img_k, self.shape, self.geom, False, 0.0, 0.0)
+        return (p * self.to_integral).reshape(self.nd, self.ns).contiguous()
+
+    def points(self, mu_xyz):
+        m = self.g.scatter_points(mu_xyz.detach().cpu().numpy(), self.center, self.size, self.dev)
+        return m.contiguous()
+
+    def __call__(self, emission_xyz, mu_xyz):
+        """(n_rings, n_angles, n_radial) float32 scatter sinogram on the GPU."""
+        em = self.line_integrals(emission_xyz.to(self.dev, torch.float32))
+        mu = self.line_integrals(mu_xyz.to(self.dev, torch.float32))
+        pts = self.points(mu_xyz)
+        sref = stream_handle(self.dev)
+        check(lib().srf_sss_prepare(self.det.data_ptr(), self.nd, pts.data_ptr(), self.ns, em.data_ptr(),
+                                    mu.data_ptr(), self.rows.data_ptr(), self.rows.numel() * 4, sref),
+              "srf_sss_prepare")
+        out = torch.empty(self.bins.shape[0], dtype=torch.float32, device=self.dev)
+        check(lib().srf_sss_sinogram(self.rows.data_ptr(), self.nd, pts.data_ptr(), self.ns, self.bins.data_ptr(),
+                                     self.bins.shape[0], self.tables.data_ptr(), float(self.scale), out.data_ptr(),
+                                     sref), "srf_sss_sinogram")
+        return out.reshape(self.g.n_rings, self.g.n_angles, self.g.n_radial)
+
+    def inputs_for_oracle(self, emission_xyz, mu_xyz):
+        """The tensors the SSS kernel consumes, on the host (tests)."""
+        em = self.line_integrals(emission_xyz.to(self.dev, torch.float32)).cpu().numpy()
+        mu = self.line_integrals(mu_xyz.to(self.dev, torch.float32)).cpu().numpy()
+        pts = self.points(mu_xyz).cpu().numpy()
+        return self.det.cpu().numpy(), pts, em, mu, self.bins.cpu().numpy()
+
+
+@nef_class
+class ScatterCorrect:
+    """Scatter correction with srfnef's fields (scatter.py:18-55).
+
+    __call__(emission_image, u_map, scanner, listmode) -> Listmode: the
+    listmode values minus the SSS estimate at each event's sinogram bin,
+    normalised so the subtracted total is scatter_fraction of the counts
+    (scatter.py:52-53).  `scanner` is unused (the geometry field describes the
+    virtual detectors); the fourth argument is the listmode, as
+    MlemFull/MlemFullTof pass it (mlem_full_tof.py:118)."""
+    low_energy: float = 400.0
+    high_energy: float = 650.0
+    resolution: float = 0.15
+    scatter_fraction: float = 0.3
+    geometry: SssGeometry = None
+
+    def estimate(self, emission_image, u_map):
+        g = self.geometry or SssGeometry()
+        g = g.update(low_energy=self.low_energy, high_energy=self.high_energy, resolution=self.resolution)
+        sss = SingleScatterSimulation(g, emission_image.shape, emission_image.center, emission_image.size)
+        em = torch.as_tensor(np.asarray(emission_image.data) if not isinstance(emission_image.data, torch.Tensor)
+                             else emission_image.data, dtype=torch.float32)
+        mu = torch.as_tensor(np.asarray(u_map.data) if not isinstance(u_map.data, torch.Tensor) else u_map.data,
+                             dtype=torch.float32)
+        return sss(em, mu), g
+
+    def __call__(self, emission_image, u_map, scanner=None, listmode=None):
+        sino, g = self.estimate(emission_image, u_map)
+        lors = torch.as_tensor(np.asarray(listmode.lors.data) if not isinstance(listmode.lors.data, torch.Tensor)
+                               else listmode.lors.data).to(sino.device, torch.float32)
+        y = torch.as_tensor(np.asarray(listmode.data) if not isinstance(listmode.data, torch.Tensor)
+                            else listmode.data).to(sino.device, torch.float32).reshape(-1)
+        s_e = sino.reshape(-1)[g.bin_of_lors(lors)]
+        tot = s_e.sum()
+        corr = y - (s_e / tot * y.sum() * self.scatter_fraction if float(tot) > 0 else 0.0)
+        data = corr if isinstance(listmode.data, torch.Tensor) else corr.cpu().numpy()
+        return Listmode(data, listmode.lors)
+
+
+__all__ = ["SssGeometry", "SingleScatterSimulation", "ScatterCorrect", "energy_tables", "sss_scale"]
