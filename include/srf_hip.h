@@ -189,6 +189,63 @@ int srf_sss_sinogram(const void *rows, int32_t n_det, const float *points, int32
                      const int32_t *bins, int32_t n_bins, const float *tables, float scale, float *out,
                      void *stream);
 
+/* ---- Scatter surrogate CNN (BF16 MFMA, FP32 accumulation) -------------------
+ * No counterpart in the reference: these train and run the 3D encoder-decoder
+ * that stands in for srf_sss_sinogram (ScatterCorrect's estimate,
+ * srfnef/corrections/scattering/scatter.py:25-55) inside the EM loop.
+ * Tensors are channels-last, act[v][c] with v = (z*H + y)*W + x; bf16 tensors
+ * are uint16 bit patterns.  Padded channel counts: inputs (cin_p, row stride of
+ * x) are multiples of 8 with zero padding; weight tensors are [taps][cout_p][cin_p]
+ * with cout_p a multiple of 32 and zero rows/columns beyond the real counts.
+ * taps is 27 (3x3x3, zero padding, stride 1) or 1 (1x1x1). */
+
+/* y[v][out_off + n] = act(bias[n] + sum_t sum_c x[v + o_t][c] w[t][n][c]) for
+ * n < cout; act = leaky ReLU of `slope` (slope < 0: identity).  Writes y_bf16
+ * and/or y_f32 (row stride out_stride).  bias may be NULL. */
+int srf_cnn_conv(const void *x, const void *w, const float *bias, int32_t D, int32_t H, int32_t W,
+                 int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, int32_t out_stride, int32_t out_off,
+                 float slope, void *y_bf16, float *y_f32, void *stream);
+/* dw[t][n][c] += sum_v dy[v][n] x[v + o_t][c]   (dy bf16 rows of dy_stride) */
+int srf_cnn_conv_wgrad(const void *x, const void *dy, int32_t dy_stride, int32_t D, int32_t H, int32_t W,
+                       int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, float *dw, void *stream);
+/* wf[t][c][n] = w[taps-1-t][n][c]: the kernel whose srf_cnn_conv of dy is the
+ * gradient with respect to x (wf is [taps][fcout_p][fcin_p]). */
+int srf_cnn_flip_weights(const void *w, int32_t taps, int32_t cout, int32_t cout_p, int32_t cin_p, int32_t cin,
+                         void *wf, int32_t fcout_p, int32_t fcin_p, void *stream);
+/* 2x2x2 average pooling (bf16) and its gradient (f32; written, not added). */
+int srf_cnn_pool(const void *x, int32_t D, int32_t H, int32_t W, int32_t C, int32_t xs, void *y, int32_t ys,
+                 void *stream);
+int srf_cnn_pool_bwd(const float *dy, int32_t D, int32_t H, int32_t W, int32_t C, int32_t dys, float *dx,
+                     int32_t dxs, void *stream);
+/* Nearest 2x upsampling of x (D,H,W) into channels [off, off+C) of y, and its
+ * gradient (sum over the 8 children, f32, written). */
+int srf_cnn_upsample(const void *x, int32_t D, int32_t H, int32_t W, int32_t C, int32_t xs, void *y, int32_t ys,
+                     int32_t off, void *stream);
+int srf_cnn_upsample_bwd(const float *dy, int32_t D, int32_t H, int32_t W, int32_t C, int32_t dys, int32_t off,
+                         float *dx, int32_t dxs, void *stream);
+/* y[v][yoff + c] (=|+=) x[v][xoff + c], c < C; f32 != 0: float tensors (only
+ * those accumulate), else bf16. */
+int srf_cnn_copy_channels(const void *x, int64_t V, int32_t C, int32_t xs, int32_t xoff, void *y, int32_t ys,
+                          int32_t yoff, int32_t f32, int32_t accumulate, void *stream);
+/* dp[v][c] = bf16(da[v][c] * (a[v][c] > 0 ? 1 : slope)) for c < C, 0 for
+ * C <= c < Cp (slope < 0: no activation). */
+int srf_cnn_act_bwd(const float *da, int32_t das, const void *a, int32_t as, int64_t V, int32_t C, int32_t Cp,
+                    float slope, void *dp, void *stream);
+/* db[c] += sum_v dp[v][c] for c < C (dp bf16 rows of Cp). */
+int srf_cnn_bias_grad(const void *dp, int64_t V, int32_t C, int32_t Cp, float *db, void *stream);
+/* loss[0] += mean((p - t)^2); grad = 2 (p - t) / n. */
+int srf_cnn_mse(const float *p, const float *t, int64_t n, float *grad, float *loss, void *stream);
+/* One Adam step (Kingma & Ba, 2015) on FP32 master weights w with gradient
+ * g * gscale; step counts from 1; wbf (may be NULL) receives bf16(w). */
+int srf_cnn_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps,
+                 int32_t step, float gscale, void *wbf, void *stream);
+/* Surrogate input: x[i][0..7] = bf16 of [e, exp(-mu), e exp(-mu), mu/4, 0, 0, 0, 0]
+ * with e = em_int[i] * scale[0] (scale is a device pointer). */
+int srf_cnn_features(const float *em_int, const float *mu_int, int64_t n, const float *scale, void *x,
+                     void *stream);
+/* f32 <-> bf16 (round to nearest even). */
+int srf_cnn_cast(const void *x, int64_t n, int32_t to_bf16, void *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

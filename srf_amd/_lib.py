@@ -67,6 +67,21 @@ SIGNATURES = [
     ("srf_sss_rows_bytes", ctypes.c_size_t, [_I32, _I32]),
     ("srf_sss_prepare", ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _P, ctypes.c_size_t, _P]),
     ("srf_sss_sinogram", ctypes.c_int, [_P, _I32, _P, _I32, _P, _I32, _P, _F, _P, _P]),
+    ("srf_cnn_conv", ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _F, _P, _P,
+                                    _P]),
+    ("srf_cnn_conv_wgrad", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P]),
+    ("srf_cnn_flip_weights", ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I32, _I32, _P]),
+    ("srf_cnn_pool", ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P]),
+    ("srf_cnn_pool_bwd", ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P]),
+    ("srf_cnn_upsample", ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I32, _I32, _P]),
+    ("srf_cnn_upsample_bwd", ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P]),
+    ("srf_cnn_copy_channels", ctypes.c_int, [_P, _I64, _I32, _I32, _I32, _P, _I32, _I32, _I32, _I32, _P]),
+    ("srf_cnn_act_bwd", ctypes.c_int, [_P, _I32, _P, _I32, _I64, _I32, _I32, _F, _P, _P]),
+    ("srf_cnn_bias_grad", ctypes.c_int, [_P, _I64, _I32, _I32, _P, _P]),
+    ("srf_cnn_mse", ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    ("srf_cnn_adam", ctypes.c_int, [_P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _F, _P, _P]),
+    ("srf_cnn_features", ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    ("srf_cnn_cast", ctypes.c_int, [_P, _I64, _I32, _P, _P]),
 ]
 
 _lib = None

@@ -71,6 +71,47 @@ def random_ellipsoid_phantom(shape, voxel, seed, body_axes=(180.0, 120.0, 120.0)
     return act, mu, center, size
 
 
+MU_WATER = 0.0096
+MU_BONE = 0.0172
+
+
+def scatter_phantom(shape, voxel, seed, n_range=(2, 10), device="cpu"):
+    """Emission and mu-map pair for the scatter surrogate: a water ellipsoid
+    body of random size and offset (activity 1, mu water) holding 2-10
+    ellipsoids, each hot tissue (activity U(0.5, 5), mu water), bone (activity
+    U(0.1, 1), mu 0.0172/mm) or air (activity 0, mu 0), placed fully inside
+    the body.  Returns (activity, mu, center, size), (nx, ny, nz) float32."""
+    rng = np.random.default_rng(seed)
+    size = [shape[0] * voxel, shape[1] * voxel, shape[2] * voxel]
+    center = [0.0, 0.0, 0.0]
+    lim = [s / 2 - voxel for s in size]
+    ba = [min(rng.uniform(130.0, 200.0), lim[0]), min(rng.uniform(90.0, 150.0), lim[1]),
+          min(rng.uniform(80.0, 130.0), lim[2])]
+    bc = [rng.uniform(-1, 1) * min(20.0, lim[0] - ba[0]), rng.uniform(-1, 1) * min(20.0, lim[1] - ba[1]), 0.0]
+    body = ellipsoid_mask(shape, center, size, bc, ba, device)
+    act = body.to(torch.float32)
+    mu = body.to(torch.float32) * MU_WATER
+    for _ in range(int(rng.integers(n_range[0], n_range[1] + 1))):
+        kind = rng.choice(3, p=[0.6, 0.25, 0.15])
+        axes = rng.uniform(10.0, 60.0, 3) / 2 if kind != 2 else rng.uniform(30.0, 100.0, 3) / 2
+        axes = np.minimum(axes, 0.8 * np.asarray(ba))
+        for _try in range(100):
+            c = np.asarray(bc) + rng.uniform(-1, 1, 3) * (np.asarray(ba) - axes)
+            if np.sum(((np.abs(c - np.asarray(bc)) + axes) / np.asarray(ba)) ** 2) <= 1.0:
+                break
+        m = ellipsoid_mask(shape, center, size, tuple(c), tuple(axes), device)
+        if kind == 0:
+            act[m] = float(rng.uniform(0.5, 5.0))
+            mu[m] = MU_WATER
+        elif kind == 1:
+            act[m] = float(rng.uniform(0.1, 1.0))
+            mu[m] = MU_BONE
+        else:
+            act[m] = 0.0
+            mu[m] = 0.0
+    return act, mu, center, size
+
+
 def sample_events(activity, center, size, n, radius, axial_length, seed, tof_fwhm_mm=None, device="cpu",
                   chunk=1 << 23):
     """n list-mode events (n, 6|7) float32 drawn from `activity` (nx,ny,nz).

@@ -145,72 +145,117 @@ class SssGeometry:
 class SingleScatterSimulation:
     """SSS sinogram of (emission, mu) images on a fixed geometry.
 
-    The (detector, scatter point) LOR set depends only on the geometry and the
-    image grid, so it is binned once into a projector plan and reused for
-    every (emission, mu) pair; scatter points follow the mu map of each call."""
+    compact=True (default) keeps only the scatter points with mu > 0 for each
+    call (the sum skips the others, so the sinogram is the same) and builds
+    the (detector, point) LORs for those; compact=False projects the full grid
+    and, with tiled=True, bins that fixed LOR set once into a projector plan.
+    rings: evaluate only the bins of these rings (their detectors only);
+    the output is then (len(rings), n_angles, n_radial)."""
 
-    def __init__(self, geometry, shape, center, size, physical_kn=False, tiled=False):
+    def __init__(self, geometry, shape, center, size, physical_kn=False, tiled=False, compact=True, rings=None):
         self.g = geometry
         self.shape = [int(v) for v in shape]
         self.center = [float(v) for v in center]
         self.size = [float(v) for v in size]
         self.geom = self.center + self.size
         self.dev = torch.device("cuda", torch.cuda.current_device())
-        self.det = geometry.detectors(self.dev)
+        self.compact = bool(compact)
+        det = geometry.detectors(self.dev)
+        bins = geometry.bin_detectors().astype(np.int64)
+        if rings is not None:
+            rings = [int(r) for r in rings]
+            if any(r < 0 or r >= geometry.n_rings for r in rings):
+                raise ValueError(f"rings {rings} outside 0..{geometry.n_rings - 1}")
+            det = det.reshape(geometry.n_rings, geometry.n_det, 4)[rings].reshape(-1, 4)
+            per = geometry.n_angles * geometry.n_radial
+            bins = np.concatenate([bins[r * per:(r + 1) * per] - r * geometry.n_det + k * geometry.n_det
+                                   for k, r in enumerate(rings)])
+        self.rings = rings
+        self.det = det.contiguous()
         self.nd = self.det.shape[0]
+        self.n_out_rings = len(rings) if rings is not None else geometry.n_rings
+        self.dx_ = self.size[0] / self.shape[0]
         # the full scatter grid (mu decides which points count)
         grid = geometry.scatter_points(np.zeros(self.shape, np.float32), self.center, self.size, self.dev)
         self.pts_xyz = grid[:, :3].contiguous()
         self.ns = self.pts_xyz.shape[0]
-        d = self.det[:, None, :3].expand(self.nd, self.ns, 3)
-        sp = self.pts_xyz[None, :, :].expand(self.nd, self.ns, 3)
-        # p1 = detector, p2 = scatter point: the projector's segment cut ends the
-        # integral at the point (siddon.cu.cc:56-57; preprocess.py:46-54 order)
-        self.lors = torch.cat([d, sp], -1).reshape(-1, 6).contiguous()
-        L2 = ((self.lors[:, 3:6] - self.lors[:, 0:3]) ** 2).sum(1)
-        dx_ = self.size[0] / self.shape[0]
-        self.to_integral = (L2 / (dx_ * dx_)).contiguous()     # Project * L^2 / dx_^2 = integral
-        self.plan = ops.Plan(self.lors, self.shape, self.geom, False) if tiled else None
-        self.bins = torch.from_numpy(geometry.bin_detectors()).to(self.dev).contiguous()
+        self.plan = None
+        if not self.compact:
+            # p1 = detector, p2 = scatter point: the projector's segment cut ends the
+            # integral at the point (siddon.cu.cc:56-57; preprocess.py:46-54 order)
+            self.lors, self.to_integral = self._lors(self.pts_xyz)
+            self.plan = ops.Plan(self.lors, self.shape, self.geom, False) if tiled else None
+        self.bins = torch.from_numpy(bins.astype(np.int32)).to(self.dev).contiguous()
         self.tables = torch.from_numpy(energy_tables(geometry.low_energy, geometry.high_energy,
                                                      geometry.resolution, physical_kn)).to(self.dev)
         self.scale = sss_scale(geometry.low_energy, geometry.high_energy, geometry.resolution)
-        self.rows = torch.empty(int(lib().srf_sss_rows_bytes(self.nd, self.ns)) // 4 + 4, dtype=torch.float32,
-                                device=self.dev)
+        self._rows = None
 
-    def line_integrals(self, img_xyz):
+    def _lors(self, pts_xyz):
+        ns = pts_xyz.shape[0]
+        d = self.det[:, None, :3].expand(self.nd, ns, 3)
+        sp = pts_xyz[None, :, :].expand(self.nd, ns, 3)
+        lors = torch.cat([d, sp], -1).reshape(-1, 6).contiguous()
+        L2 = ((lors[:, 3:6] - lors[:, 0:3]) ** 2).sum(1)
+        return lors, (L2 / (self.dx_ * self.dx_)).contiguous()     # Project * L^2 / dx_^2 = integral
+
+    def _integrals(self, img_xyz, lors, to_integral, plan=None):
         img_k = ops.to_kernel_layout(img_xyz.contiguous())
-        if self.plan is not None:
-            p = self.plan.project(img_k)
+        if plan is not None:
+            p = plan.project(img_k)
         else:
-            p = ops.project(self.lors, img_k, self.shape, self.geom, False, 0.0, 0.0)
-        return (p * self.to_integral).reshape(self.nd, self.ns).contiguous()
+            p = ops.project(lors, img_k, self.shape, self.geom, False, 0.0, 0.0)
+        return (p * to_integral).reshape(self.nd, -1).contiguous()
+
+    def line_integrals(self, img_xyz, pts=None):
+        """(nd, ns) integrals from each detector to each point of `pts`
+        ((ns, 4) or (ns, 3); default: the full grid)."""
+        img_xyz = img_xyz.to(self.dev, torch.float32)
+        if pts is None and not self.compact:
+            return self._integrals(img_xyz, self.lors, self.to_integral, self.plan)
+        xyz = self.pts_xyz if pts is None else pts[:, :3].contiguous()
+        lors, ti = self._lors(xyz)
+        return self._integrals(img_xyz, lors, ti)
 
     def points(self, mu_xyz):
         m = self.g.scatter_points(mu_xyz.detach().cpu().numpy(), self.center, self.size, self.dev)
+        if self.compact:
+            m = m[m[:, 3] != 0]
         return m.contiguous()
 
+    def _rows_for(self, ns):
+        need = int(lib().srf_sss_rows_bytes(self.nd, max(ns, 1))) // 4 + 4
+        if self._rows is None or self._rows.numel() < need:
+            self._rows = torch.empty(need, dtype=torch.float32, device=self.dev)
+        return self._rows
+
     def __call__(self, emission_xyz, mu_xyz):
-        """(n_rings, n_angles, n_radial) float32 scatter sinogram on the GPU."""
-        em = self.line_integrals(emission_xyz.to(self.dev, torch.float32))
-        mu = self.line_integrals(mu_xyz.to(self.dev, torch.float32))
+        """(n_rings or len(rings), n_angles, n_radial) float32 scatter sinogram."""
         pts = self.points(mu_xyz)
+        out = torch.zeros(self.bins.shape[0], dtype=torch.float32, device=self.dev)
+        shape = (self.n_out_rings, self.g.n_angles, self.g.n_radial)
+        ns = pts.shape[0]
+        if ns == 0:
+            return out.reshape(shape)
+        sel = None if not self.compact else pts
+        em = self.line_integrals(emission_xyz, sel)
+        mu = self.line_integrals(mu_xyz, sel)
+        rows = self._rows_for(ns)
         sref = stream_handle(self.dev)
-        check(lib().srf_sss_prepare(self.det.data_ptr(), self.nd, pts.data_ptr(), self.ns, em.data_ptr(),
-                                    mu.data_ptr(), self.rows.data_ptr(), self.rows.numel() * 4, sref),
-              "srf_sss_prepare")
-        out = torch.empty(self.bins.shape[0], dtype=torch.float32, device=self.dev)
-        check(lib().srf_sss_sinogram(self.rows.data_ptr(), self.nd, pts.data_ptr(), self.ns, self.bins.data_ptr(),
+        check(lib().srf_sss_prepare(self.det.data_ptr(), self.nd, pts.data_ptr(), ns, em.data_ptr(),
+                                    mu.data_ptr(), rows.data_ptr(), rows.numel() * 4, sref), "srf_sss_prepare")
+        check(lib().srf_sss_sinogram(rows.data_ptr(), self.nd, pts.data_ptr(), ns, self.bins.data_ptr(),
                                      self.bins.shape[0], self.tables.data_ptr(), float(self.scale), out.data_ptr(),
                                      sref), "srf_sss_sinogram")
-        return out.reshape(self.g.n_rings, self.g.n_angles, self.g.n_radial)
+        return out.reshape(shape)
 
     def inputs_for_oracle(self, emission_xyz, mu_xyz):
         """The tensors the SSS kernel consumes, on the host (tests)."""
-        em = self.line_integrals(emission_xyz.to(self.dev, torch.float32)).cpu().numpy()
-        mu = self.line_integrals(mu_xyz.to(self.dev, torch.float32)).cpu().numpy()
-        pts = self.points(mu_xyz).cpu().numpy()
-        return self.det.cpu().numpy(), pts, em, mu, self.bins.cpu().numpy()
+        pts = self.points(mu_xyz)
+        sel = None if not self.compact else pts
+        em = self.line_integrals(emission_xyz, sel).cpu().numpy()
+        mu = self.line_integrals(mu_xyz, sel).cpu().numpy()
+        return self.det.cpu().numpy(), pts.cpu().numpy(), em, mu, self.bins.cpu().numpy()
 
 
 @nef_class

@@ -1,0 +1,412 @@
+"""Learned scatter surrogate: a 3D encoder-decoder CNN that maps (emission,
+mu-map) to the coarse SSS sinogram, trained and run by the HIP kernels of
+srf_cnn.hip (BF16 storage, MFMA with FP32 accumulation, Adam on FP32 masters).
+
+The reference has no learned part; this stands in for ScatterCorrect's SSS
+estimate (srfnef/corrections/scattering/scatter.py:25-55) inside the EM loop,
+with plain SSS as the fallback and the judge:
+
+  features   the emission and attenuation line integrals along every sinogram
+             bin's LOR (the projector over the 131072 bin LORs), assembled per
+             bin as [e, exp(-mu), e exp(-mu), mu/4] with e normalised by its
+             mean m; the sinogram (rings, angles, radial) is the CNN's 3D grid
+  network    U-Net: per level two 3x3x3 convolutions + leaky ReLU (16, 32, 64,
+             128 channels), 2x2x2 average pooling down, nearest upsampling and
+             skip concatenation up, a 1x1x1 linear head
+  output     SSS ~= m / kappa * net(features): scatter is linear in the
+             emission, so the network sees and predicts normalised quantities
+  check      SSS evaluated on a few rings (their detectors only); when the
+             surrogate's relative L2 error there exceeds the tolerance (2%), the
+             full SSS is run and returned instead
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from ._lib import check, lib, stream_handle
+from . import ops
+from .sss import SingleScatterSimulation, SssGeometry
+
+SLOPE = 0.01
+
+
+def _c8(n):
+    return (n + 7) // 8 * 8
+
+
+def _c32(n):
+    return (n + 31) // 32 * 32
+
+
+def unet_layers(in_ch, widths):
+    """[(name, taps, cin, cout)] in parameter order."""
+    L, c = [], in_ch
+    for l, w in enumerate(widths):
+        L += [(f"enc{l}a", 27, c, w), (f"enc{l}b", 27, w, w)]
+        c = w
+    for l in range(len(widths) - 2, -1, -1):
+        L += [(f"dec{l}a", 27, widths[l + 1] + widths[l], widths[l]), (f"dec{l}b", 27, widths[l], widths[l])]
+    L += [("head", 1, widths[0], 1)]
+    return L
+
+
+def bf16_bits_to_float(t):
+    """int16 tensor of bf16 bit patterns -> float32 tensor (same device)."""
+    return (t.to(torch.int32) << 16).view(torch.float32)
+
+
+class _Layer:
+    def __init__(self, name, taps, cin, cout, dev):
+        self.name, self.taps, self.cin, self.cout = name, taps, cin, cout
+        self.cin_p, self.cout_p = _c8(cin), _c32(cout)
+        self.fcin_p, self.fcout_p = _c8(cout), _c32(cin)      # flipped (input-gradient) kernel
+        wshape = (taps, self.cout_p, self.cin_p)
+        z = lambda *s, dt=torch.float32: torch.zeros(s, dtype=dt, device=dev)
+        self.w, self.gw, self.mw, self.vw = z(*wshape), z(*wshape), z(*wshape), z(*wshape)
+        self.wbf = z(*wshape, dt=torch.int16)
+        self.wf = z(taps, self.fcout_p, self.fcin_p, dt=torch.int16)
+        self.b, self.gb, self.mb, self.vb = z(cout), z(cout), z(cout), z(cout)
+
+
+class UNet:
+    """The surrogate CNN on the GPU.  Every arithmetic step is a HIP kernel of
+    libsrf_hip.so; torch provides device memory, zero fills and the stream."""
+
+    def __init__(self, in_ch=4, widths=(16, 32, 64, 128), slope=SLOPE, seed=0, device=None):
+        self.dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.in_ch, self.widths, self.slope = in_ch, tuple(widths), float(slope)
+        self.layers = [_Layer(n, t, ci, co, self.dev) for n, t, ci, co in unet_layers(in_ch, widths)]
+        self.L = {l.name: l for l in self.layers}
+        self.step_count = 0
+        self._bufs = {}
+        self.init(seed)
+
+    # ------------------------------------------------------------ parameters
+    def init(self, seed):
+        """He-normal weights (std sqrt(2 / (taps cin))), zero biases, head std 1e-2."""
+        rng = np.random.default_rng(seed)
+        p = {}
+        for l in self.layers:
+            std = math.sqrt(2.0 / (l.taps * l.cin)) if l.name != "head" else 1e-2
+            p[l.name] = (rng.normal(0.0, std, (l.taps, l.cout, l.cin)), np.zeros(l.cout))
+        self.set_params(p)
+
+    def set_params(self, params):
+        """params: {name: (w (taps, cout, cin), b (cout,))} (numpy or torch)."""
+        for l in self.layers:
+            w, b = params[l.name]
+            l.w.zero_()
+            l.w[:, :l.cout, :l.cin] = torch.as_tensor(np.asarray(w, np.float32)).to(self.dev)
+            l.b.copy_(torch.as_tensor(np.asarray(b, np.float32)).to(self.dev))
+            for t in (l.mw, l.vw, l.mb, l.vb):
+                t.zero_()
+        self.step_count = 0
+        self._sync_weights()
+
+    def params(self):
+        return {l.name: (l.w[:, :l.cout, :l.cin].cpu().numpy().copy(), l.b.cpu().numpy().copy())
+                for l in self.layers}
+
+    def grads(self):
+        return {l.name: (l.gw[:, :l.cout, :l.cin].cpu().numpy().copy(), l.gb.cpu().numpy().copy())
+                for l in self.layers}
+
+    def zero_grad(self):
+        for l in self.layers:
+            l.gw.zero_()
+            l.gb.zero_()
+
+    def _sync_weights(self):
+        s = stream_handle(self.dev)
+        for l in self.layers:
+            check(lib().srf_cnn_cast(l.w.data_ptr(), l.w.numel(), 1, l.wbf.data_ptr(), s), "srf_cnn_cast")
+            check(lib().srf_cnn_flip_weights(l.wbf.data_ptr(), l.taps, l.cout, l.cout_p, l.cin_p, l.cin,
+                                             l.wf.data_ptr(), l.fcout_p, l.fcin_p, s), "srf_cnn_flip_weights")
+
+    def save(self, path, **extra):
+        """Compressed .npz of the inference weights: the bf16 values the
+        convolutions use (as float32) and the float32 biases, plus metadata."""
+        d = {}
+        for l in self.layers:
+            d[l.name + ".w"] = bf16_bits_to_float(l.wbf)[:, :l.cout, :l.cin].cpu().numpy().copy()
+            d[l.name + ".b"] = l.b.cpu().numpy().copy()
+        meta = dict(in_ch=self.in_ch, widths=list(self.widths), slope=self.slope, **extra)
+        d["meta"] = np.frombuffer(repr(meta).encode(), dtype=np.uint8)
+        np.savez_compressed(path, **d)
+
+    @classmethod
+    def load(cls, path, device=None):
+        z = np.load(path, allow_pickle=False)
+        meta = _parse_meta(bytes(z["meta"]).decode())
+        net = cls(meta["in_ch"], tuple(meta["widths"]), meta["slope"], device=device)
+        net.set_params({l.name: (z[l.name + ".w"], z[l.name + ".b"]) for l in net.layers})
+        return net, meta
+
+    # --------------------------------------------------------------- buffers
+    def _buffers(self, D, H, W):
+        key = (D, H, W)
+        if key in self._bufs:
+            return self._bufs[key]
+        n = len(self.widths)
+        if D % (1 << (n - 1)) or H % (1 << (n - 1)) or W % (1 << (n - 1)):
+            raise ValueError(f"grid {key} must be divisible by {1 << (n - 1)}")
+        dims = [(D >> l, H >> l, W >> l) for l in range(n)]
+        V = [d[0] * d[1] * d[2] for d in dims]
+        bf = lambda *s: torch.zeros(s, dtype=torch.int16, device=self.dev)
+        f32 = lambda *s: torch.zeros(s, dtype=torch.float32, device=self.dev)
+        w = self.widths
+        B = {"dims": dims, "V": V}
+        for l in range(n):
+            B[f"ea{l}"], B[f"e{l}"] = bf(V[l], w[l]), bf(V[l], w[l])
+            B[f"gea{l}"], B[f"ge{l}"] = f32(V[l], w[l]), f32(V[l], w[l])
+            cmax = max(_c8(c) for c in [w[l], 1] + ([w[l + 1]] if l + 1 < n else []))
+            B[f"dp{l}"] = bf(V[l], max(cmax, 8))
+            if l + 1 < n:
+                B[f"p{l}"] = bf(V[l + 1], w[l])
+                B[f"gp{l}"] = f32(V[l + 1], w[l])
+                cu = w[l + 1] + w[l]
+                B[f"u{l}"], B[f"gu{l}"] = bf(V[l], cu), f32(V[l], cu)
+                B[f"da{l}"], B[f"d{l}"] = bf(V[l], w[l]), bf(V[l], w[l])
+                B[f"gda{l}"], B[f"gd{l}"] = f32(V[l], w[l]), f32(V[l], w[l])
+        B["out"] = f32(V[0])
+        B["gout"] = f32(V[0])
+        B["loss"] = f32(1)
+        self._bufs[key] = B
+        return B
+
+    # --------------------------------------------------------------- kernels
+    def _conv(self, L, x, dims, y_bf=None, y_f32=None, stride=None, off=0, act=True):
+        D, H, W = dims
+        check(lib().srf_cnn_conv(x.data_ptr(), L.wbf.data_ptr(), L.b.data_ptr(), D, H, W, L.cin_p, L.cout, L.cout_p,
+                                 L.taps, stride or L.cout, off, self.slope if act else -1.0,
+                                 y_bf.data_ptr() if y_bf is not None else None,
+                                 y_f32.data_ptr() if y_f32 is not None else None, stream_handle(self.dev)),
+              "srf_cnn_conv " + L.name)
+
+    def _back(self, L, x, a, ga, ga_stride, dims, dp, gx=None, act=True):
+        """Gradients of layer L (input x, output a, dL/da = ga); gx <- dL/dx."""
+        D, H, W = dims
+        V = D * H * W
+        s = stream_handle(self.dev)
+        check(lib().srf_cnn_act_bwd(ga.data_ptr(), ga_stride, a.data_ptr(), L.cout, V, L.cout, L.fcin_p,
+                                    self.slope if act else -1.0, dp.data_ptr(), s), "srf_cnn_act_bwd")
+        check(lib().srf_cnn_conv_wgrad(x.data_ptr(), dp.data_ptr(), L.fcin_p, D, H, W, L.cin_p, L.cout, L.cout_p,
+                                       L.taps, L.gw.data_ptr(), s), "srf_cnn_conv_wgrad " + L.name)
+        check(lib().srf_cnn_bias_grad(dp.data_ptr(), V, L.cout, L.fcin_p, L.gb.data_ptr(), s), "srf_cnn_bias_grad")
+        if gx is not None:
+            check(lib().srf_cnn_conv(dp.data_ptr(), L.wf.data_ptr(), None, D, H, W, L.fcin_p, L.cin, L.fcout_p,
+                                     L.taps, L.cin, 0, -1.0, None, gx.data_ptr(), s), "srf_cnn_conv dgrad " + L.name)
+
+    # ------------------------------------------------------------------ run
+    def forward(self, x0, dims):
+        """x0: int16 (V, 8) bf16 features (channels-last); dims (D, H, W).
+        Returns the float32 (V,) output buffer (reused by the next call)."""
+        B = self._buffers(*dims)
+        n, w, s = len(self.widths), self.widths, stream_handle(self.dev)
+        h = x0
+        for l in range(n):
+            d = B["dims"][l]
+            if l > 0:
+                pd = B["dims"][l - 1]
+                check(lib().srf_cnn_pool(B[f"e{l - 1}"].data_ptr(), pd[0], pd[1], pd[2], w[l - 1], w[l - 1],
+                                         B[f"p{l - 1}"].data_ptr(), w[l - 1], s), "srf_cnn_pool")
+                h = B[f"p{l - 1}"]
+            self._conv(self.L[f"enc{l}a"], h, d, y_bf=B[f"ea{l}"])
+            self._conv(self.L[f"enc{l}b"], B[f"ea{l}"], d, y_bf=B[f"e{l}"])
+        prev = B[f"e{n - 1}"]
+        for l in range(n - 2, -1, -1):
+            d, cd = B["dims"][l], B["dims"][l + 1]
+            cu = w[l + 1] + w[l]
+            check(lib().srf_cnn_upsample(prev.data_ptr(), cd[0], cd[1], cd[2], w[l + 1], w[l + 1],
+                                         B[f"u{l}"].data_ptr(), cu, 0, s), "srf_cnn_upsample")
+            check(lib().srf_cnn_copy_channels(B[f"e{l}"].data_ptr(), B["V"][l], w[l], w[l], 0, B[f"u{l}"].data_ptr(),
+                                              cu, w[l + 1], 0, 0, s), "srf_cnn_copy_channels")
+            self._conv(self.L[f"dec{l}a"], B[f"u{l}"], d, y_bf=B[f"da{l}"])
+            self._conv(self.L[f"dec{l}b"], B[f"da{l}"], d, y_bf=B[f"d{l}"])
+            prev = B[f"d{l}"]
+        self._conv(self.L["head"], prev, B["dims"][0], y_f32=B["out"], stride=1, act=False)
+        self._x0 = x0
+        return B["out"]
+
+    def backward(self, gout, dims):
+        """Accumulate parameter gradients for dL/d(output) = gout (float32 (V,))."""
+        B = self._buffers(*dims)
+        n, w, s = len(self.widths), self.widths, stream_handle(self.dev)
+        dp = lambda l: B[f"dp{l}"]
+        top = B["d0"] if n > 1 else B["e0"]
+        gtop = B["gd0"] if n > 1 else B["ge0"]
+        self._back(self.L["head"], top, gout, gout, 1, B["dims"][0], dp(0), gx=gtop, act=False)
+        for l in range(0, n - 1):
+            d, cd = B["dims"][l], B["dims"][l + 1]
+            cu = w[l + 1] + w[l]
+            self._back(self.L[f"dec{l}b"], B[f"da{l}"], B[f"d{l}"], B[f"gd{l}"], w[l], d, dp(l), gx=B[f"gda{l}"])
+            self._back(self.L[f"dec{l}a"], B[f"u{l}"], B[f"da{l}"], B[f"gda{l}"], w[l], d, dp(l), gx=B[f"gu{l}"])
+            dst = B[f"gd{l + 1}"] if l + 1 < n - 1 else B[f"ge{n - 1}"]
+            check(lib().srf_cnn_upsample_bwd(B[f"gu{l}"].data_ptr(), cd[0], cd[1], cd[2], w[l + 1], cu, 0,
+                                             dst.data_ptr(), w[l + 1], s), "srf_cnn_upsample_bwd")
+        for l in range(n - 1, -1, -1):
+            d = B["dims"][l]
+            if l < n - 1:
+                cu = w[l + 1] + w[l]
+                check(lib().srf_cnn_copy_channels(B[f"gu{l}"].data_ptr(), B["V"][l], w[l], cu, w[l + 1],
+                                                  B[f"ge{l}"].data_ptr(), w[l], 0, 1, 1, s), "srf_cnn_copy_channels")
+            xin = self._x0 if l == 0 else B[f"p{l - 1}"]
+            self._back(self.L[f"enc{l}b"], B[f"ea{l}"], B[f"e{l}"], B[f"ge{l}"], w[l], d, dp(l), gx=B[f"gea{l}"])
+            self._back(self.L[f"enc{l}a"], xin, B[f"ea{l}"], B[f"gea{l}"], w[l], d, dp(l),
+                       gx=B[f"gp{l - 1}"] if l > 0 else None)
+            if l > 0:
+                pd = B["dims"][l - 1]
+                check(lib().srf_cnn_pool_bwd(B[f"gp{l - 1}"].data_ptr(), pd[0], pd[1], pd[2], w[l - 1], w[l - 1],
+                                             B[f"ge{l - 1}"].data_ptr(), w[l - 1], s), "srf_cnn_pool_bwd")
+
+    def loss_and_grad(self, out, target, dims):
+        """MSE of out vs target into the loss buffer; returns dL/dout (buffer)."""
+        B = self._buffers(*dims)
+        check(lib().srf_cnn_mse(out.data_ptr(), target.data_ptr(), out.numel(), B["gout"].data_ptr(),
+                                B["loss"].data_ptr(), stream_handle(self.dev)), "srf_cnn_mse")
+        return B["gout"]
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, gscale=1.0):
+        self.step_count += 1
+        s = stream_handle(self.dev)
+        for l in self.layers:
+            check(lib().srf_cnn_adam(l.w.data_ptr(), l.gw.data_ptr(), l.mw.data_ptr(), l.vw.data_ptr(), l.w.numel(),
+                                     lr, b1, b2, eps, self.step_count, gscale, l.wbf.data_ptr(), s), "srf_cnn_adam")
+            check(lib().srf_cnn_adam(l.b.data_ptr(), l.gb.data_ptr(), l.mb.data_ptr(), l.vb.data_ptr(), l.cout,
+                                     lr, b1, b2, eps, self.step_count, gscale, None, s), "srf_cnn_adam")
+            check(lib().srf_cnn_flip_weights(l.wbf.data_ptr(), l.taps, l.cout, l.cout_p, l.cin_p, l.cin,
+                                             l.wf.data_ptr(), l.fcout_p, l.fcin_p, s), "srf_cnn_flip_weights")
+
+
+def _parse_meta(text):
+    import ast
+    return ast.literal_eval(text)
+
+
+def to_bf16(x):
+    """float32 device tensor -> int16 bf16 bit patterns (HIP cast kernel)."""
+    x = x.contiguous()
+    y = torch.empty(x.shape, dtype=torch.int16, device=x.device)
+    check(lib().srf_cnn_cast(x.data_ptr(), x.numel(), 1, y.data_ptr(), stream_handle(x.device)), "srf_cnn_cast")
+    return y
+
+
+class SinogramFeatures:
+    """Per-bin emission and attenuation line integrals along the coarse
+    sinogram's LORs (the bins' detector pairs), assembled into the network's
+    8-channel bf16 input by srf_cnn_features."""
+
+    def __init__(self, geometry, shape, center, size, device=None):
+        self.g = geometry
+        self.shape = [int(v) for v in shape]
+        self.geom = [float(v) for v in center] + [float(v) for v in size]
+        self.dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        det = geometry.detectors(self.dev)
+        bins = torch.from_numpy(geometry.bin_detectors().astype(np.int64)).to(self.dev)
+        self.lors = torch.cat([det[bins[:, 0], :3], det[bins[:, 1], :3]], 1).contiguous()
+        L2 = ((self.lors[:, 3:6] - self.lors[:, 0:3]) ** 2).sum(1)
+        dx_ = self.geom[3] / self.shape[0]
+        self.to_integral = (L2 / (dx_ * dx_)).contiguous()
+        self.dims = (geometry.n_rings, geometry.n_angles, geometry.n_radial)
+        self.n = self.lors.shape[0]
+
+    def integrals(self, img_xyz):
+        img_k = ops.to_kernel_layout(img_xyz.to(self.dev, torch.float32).contiguous())
+        return ops.project(self.lors, img_k, self.shape, self.geom, False, 0.0, 0.0) * self.to_integral
+
+    def __call__(self, em_xyz, mu_xyz):
+        """-> (x (n, 8) int16 bf16, m (1,) float32 mean emission integral)."""
+        em = self.integrals(em_xyz)
+        mu = self.integrals(mu_xyz)
+        m = em.mean().reshape(1).clamp_min(1e-30)
+        inv = (1.0 / m).contiguous()
+        x = torch.empty((self.n, 8), dtype=torch.int16, device=self.dev)
+        check(lib().srf_cnn_features(em.data_ptr(), mu.data_ptr(), self.n, inv.data_ptr(), x.data_ptr(),
+                                     stream_handle(self.dev)), "srf_cnn_features")
+        return x, m
+
+
+def relative_l2(a, b):
+    return float(torch.linalg.vector_norm((a - b).double()) / torch.linalg.vector_norm(b.double()).clamp_min(1e-300))
+
+
+class ScatterSurrogate:
+    """SSS estimate by the trained CNN, checked against SSS on `check_rings`.
+
+    estimate(em, mu) -> (sinogram (n_rings, n_angles, n_radial), info) where
+    info = {"fallback": bool, "check_err": float}; on fallback the sinogram is
+    the full SSS."""
+
+    def __init__(self, net, kappa, geometry, shape, center, size, check_rings=(5, 16, 26), tol=0.02,
+                 physical_kn=False):
+        self.net, self.kappa, self.g, self.tol = net, float(kappa), geometry, float(tol)
+        self.features = SinogramFeatures(geometry, shape, center, size, net.dev)
+        self.sss_args = (geometry, shape, center, size)
+        self.physical_kn = physical_kn
+        self.check_rings = tuple(int(r) for r in check_rings) if check_rings else ()
+        self._check = None
+        self._full = None
+
+    def predict(self, em_xyz, mu_xyz):
+        x, m = self.features(em_xyz, mu_xyz)
+        out = self.net.forward(x, self.features.dims)
+        return (out * (m / self.kappa)).reshape(self.features.dims)
+
+    def full_sss(self, em_xyz, mu_xyz):
+        if self._full is None:
+            self._full = SingleScatterSimulation(*self.sss_args, physical_kn=self.physical_kn)
+        return self._full(em_xyz, mu_xyz)
+
+    def estimate(self, em_xyz, mu_xyz):
+        pred = self.predict(em_xyz, mu_xyz)
+        info = {"fallback": False, "check_err": None}
+        if self.check_rings:
+            if self._check is None:
+                self._check = SingleScatterSimulation(*self.sss_args, physical_kn=self.physical_kn,
+                                                      rings=self.check_rings)
+            ref = self._check(em_xyz, mu_xyz)
+            err = relative_l2(pred[list(self.check_rings)], ref)
+            info["check_err"] = err
+            if not (err <= self.tol):
+                info["fallback"] = True
+                return self.full_sss(em_xyz, mu_xyz), info
+        return pred, info
+
+
+def train(net, X, T, dims, epochs, batch=4, lr=1e-3, seed=0, log=None, lr_min_frac=0.05, time_limit=None):
+    """Adam on the MSE between net(X[i]) and T[i] (both on the GPU:
+    X (N, V, 8) int16 bf16, T (N, V) float32).  Cosine learning-rate decay to
+    lr * lr_min_frac.  Returns the per-epoch mean losses."""
+    N = X.shape[0]
+    g = torch.Generator().manual_seed(int(seed))
+    B = net._buffers(*dims)
+    steps_per_epoch = max(1, N // batch)
+    total = epochs * steps_per_epoch
+    hist, k, t0 = [], 0, time.time()
+    for ep in range(epochs):
+        perm = torch.randperm(N, generator=g)
+        B["loss"].zero_()
+        for sidx in range(steps_per_epoch):
+            net.zero_grad()
+            for i in perm[sidx * batch:(sidx + 1) * batch].tolist():
+                out = net.forward(X[i], dims)
+                gout = net.loss_and_grad(out, T[i], dims)
+                net.backward(gout, dims)
+            frac = k / max(1, total - 1)
+            cur = lr * (lr_min_frac + (1 - lr_min_frac) * 0.5 * (1 + math.cos(math.pi * frac)))
+            net.adam(cur, gscale=1.0 / batch)
+            k += 1
+        hist.append(float(B["loss"].item()) / (steps_per_epoch * batch))
+        if log:
+            log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.4e} lr {cur:.2e} {time.time() - t0:.0f}s")
+        if time_limit is not None and time.time() - t0 > time_limit:
+            if log:
+                log(f"time limit reached after {ep + 1} epochs")
+            break
+    return hist
+
+
+__all__ = ["UNet", "SinogramFeatures", "ScatterSurrogate", "train", "relative_l2", "to_bf16",
+           "bf16_bits_to_float", "unet_layers"]
