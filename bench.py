@@ -167,9 +167,12 @@ def main():
     fp_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
     bp_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
     nvox = shape[0] * shape[1] * shape[2]
-    # algorithmic bytes: LOR (24) + tof (4) + out/in values + 4 per term, + image once
-    fp_bytes = n_sub_ev * (24 + 4 + 4 + 4 * te_fp) + 4 * nvox
-    bp_bytes = n_sub_ev * (24 + 4 + 8 + 4 * te_bp) + 4 * nvox
+    # algorithmic bytes per launch: LOR (24) + tof (4) + event values + 4 per term, + the image once.
+    # Event values: on the plan path the projector reads y and writes q (8) and the backprojector
+    # reads q (4); on the event-per-lane path the projector writes p (4) and the backprojector reads
+    # y and p (8, ratio fused into its load).
+    fp_bytes = n_sub_ev * (24 + 4 + (4 if args.event_kernels else 8) + 4 * te_fp) + 4 * nvox
+    bp_bytes = n_sub_ev * (24 + 4 + (8 if args.event_kernels else 4) + 4 * te_bp) + 4 * nvox
     fp_bw = fp_bytes / (fp_ms * 1e-3)
     bp_bw = bp_bytes / (bp_ms * 1e-3)
     dom = ("backproject", bp_ms, bp_bw, bp_bytes) if bp_ms >= fp_ms else ("project", fp_ms, fp_bw, fp_bytes)

@@ -4,7 +4,8 @@
  * Conventions shared by every entry point
  *  - All pointers are device pointers (HBM) unless the comment says otherwise.
  *    The caller owns every buffer; the library allocates device memory only in
- *    srf_plan_create (freed by srf_plan_destroy).
+ *    srf_plan_create[_ex] and srf_batch_create (freed by the matching
+ *    *_destroy).
  *  - LORs are AoS rows of `lor_stride` floats [x1,y1,z1,x2,y2,z2,(tof,...)] in
  *    mm, exactly Lors.data of srfnef/data/lors.py:12-22 (stride 6 or 7).
  *  - TOF values are read from `tof_values[e * tof_stride]`; pass
@@ -147,6 +148,14 @@ typedef struct srf_plan srf_plan;
 int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride,
                     const float *tof_values, int64_t tof_stride,
                     srf_grid_t grid, const srf_tof_t *tof, srf_plan **plan, void *stream);
+/* srf_plan_create with the largest work item set to item_records segments
+ * (256..8192; srf_plan_create uses 8192).  Smaller items spread a crowded
+ * tile over more workgroups (more tile partials to sum); the batch autotuner
+ * chooses it. */
+int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride,
+                       const float *tof_values, int64_t tof_stride,
+                       srf_grid_t grid, const srf_tof_t *tof, int32_t item_records,
+                       srf_plan **plan, void *stream);
 int srf_plan_destroy(srf_plan *plan);
 int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments,
                   int64_t *n_items, size_t *device_bytes);
@@ -170,6 +179,52 @@ int srf_plan_backproject_ratio(srf_plan *plan, const float *y, const float *p,
 int srf_plan_digest(srf_plan *plan, srf_bp_mode_t mode, uint64_t *hash, int32_t *count,
                     void *stream);
 
+/* -------------------------------------------------------------- batches --
+ * Many reconstructions (scans of different sizes and event counts) as one
+ * launch per direction.  No counterpart in the reference, which reconstructs
+ * one scan per OsemFull call (srfnef/functions/osem_full.py:49-87); per scan
+ * the results are bit-identical to the single-plan calls
+ * (srf_plan_project_ratio + srf_plan_backproject(SRF_BP_REFERENCE) +
+ * srf_em_update).
+ *
+ * A batch binds one plan per scan (typically one OSEM subset each) and the
+ * scan's EM buffers.  Plans may differ in grid, TOF and voxel aspect: the work
+ * list is grouped by kernel variant, one launch per non-empty group (one
+ * launch per direction when all scans share a variant).  Every call's kernel
+ * arguments are fixed at
+ * creation, so a sequence of srf_batch_em_step calls can be captured in a HIP
+ * graph and replayed.  srf_batch_create allocates the work list (freed by
+ * srf_batch_destroy) and synchronises `stream`; plans and buffers must outlive
+ * the batch. */
+typedef struct srf_batch srf_batch;
+
+typedef struct {
+    float *image;         /* estimate x (kernel layout), updated by the EM step */
+    const float *emap;    /* sensitivity image (kernel layout) */
+    const float *y;       /* event values, one per plan event */
+    float *q;             /* workspace, one float per plan event */
+} srf_batch_scan_t;
+
+typedef struct {
+    int32_t persistent;          /* 0: one workgroup per work item; 1: a persistent
+                                  * grid that takes items from an atomic queue */
+    int32_t workgroups_per_cu;   /* persistent grid = this x compute units (1..16) */
+    int32_t order;               /* 0: scan order; 1: work items by decreasing size */
+} srf_batch_cfg_t;
+
+int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int32_t n_scans,
+                     const srf_batch_cfg_t *cfg, srf_batch **batch, void *stream);
+int srf_batch_destroy(srf_batch *batch);
+int srf_batch_info(const srf_batch *batch, int32_t *n_scans, int32_t *n_work, int32_t *grid,
+                   int64_t *n_events, size_t *device_bytes);
+/* q = ratio(y, A x) for every scan (srf_plan_project_ratio semantics). */
+int srf_batch_project_ratio(srf_batch *batch, srf_ratio_kind_t kind, float eps, void *stream);
+/* x = x * A^T q / emap for every scan (srf_plan_backproject with
+ * SRF_BP_REFERENCE, then srf_em_update with zero_guard, fused). */
+int srf_batch_backproject_update(srf_batch *batch, float zero_guard, void *stream);
+/* One EM sub-iteration for every scan: the two calls above. */
+int srf_batch_em_step(srf_batch *batch, srf_ratio_kind_t kind, float eps, float zero_guard, void *stream);
+
 /* ----------------------------------------------------- scatter estimate --
  * Single-scatter simulation on a virtual-detector sinogram: the per-LOR
  * Klein-Nishina sum of srfnef/corrections/scattering/scatter_fraction/
@@ -188,6 +243,16 @@ int srf_sss_prepare(const float *det, int32_t n_det, const float *points, int32_
 int srf_sss_sinogram(const void *rows, int32_t n_det, const float *points, int32_t n_points,
                      const int32_t *bins, int32_t n_bins, const float *tables, float scale, float *out,
                      void *stream);
+
+/* Scatter correction of list-mode values (ScatterCorrect.__call__,
+ * srfnef/corrections/scattering/scatter.py:51-53):
+ *   out[e] = (y[e] - sino[bins[e]] / S * Y * fraction) * weight[e]
+ * with S = sum_e sino[bins[e]], Y = sum_e y[e] (double sums in work[2]);
+ * y NULL = ones, weight NULL = ones (pass 1/attenuation to fold in the
+ * attenuation correction); bins outside [0, n_bins) count as zero scatter.
+ * S = 0 subtracts nothing.  out may alias y.  No memset: graph-safe. */
+int srf_scatter_correct(const float *y, const float *weight, const int32_t *bins, const float *sino, int64_t n_bins,
+                        int64_t n, float fraction, double *work, float *out, void *stream);
 
 /* ---- Scatter surrogate CNN (BF16 MFMA, FP32 accumulation) -------------------
  * No counterpart in the reference: these train and run the 3D encoder-decoder

@@ -31,6 +31,15 @@ class SrfTof(ctypes.Structure):
     _fields_ = [("tof_bin", ctypes.c_float), ("tof_sigma2", ctypes.c_float)]
 
 
+class SrfBatchScan(ctypes.Structure):
+    _fields_ = [("image", ctypes.c_void_p), ("emap", ctypes.c_void_p), ("y", ctypes.c_void_p),
+                ("q", ctypes.c_void_p)]
+
+
+class SrfBatchCfg(ctypes.Structure):
+    _fields_ = [("persistent", ctypes.c_int32), ("workgroups_per_cu", ctypes.c_int32), ("order", ctypes.c_int32)]
+
+
 class SrfError(RuntimeError):
     pass
 
@@ -55,7 +64,17 @@ SIGNATURES = [
     ("srf_image_to_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
     ("srf_image_from_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
     ("srf_plan_create", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(_P), _P]),
+    ("srf_plan_create_ex", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, _I32, ctypes.POINTER(_P),
+                                          _P]),
     ("srf_plan_destroy", ctypes.c_int, [_P]),
+    ("srf_batch_create", ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(SrfBatchScan), _I32,
+                                        ctypes.POINTER(SrfBatchCfg), ctypes.POINTER(_P), _P]),
+    ("srf_batch_destroy", ctypes.c_int, [_P]),
+    ("srf_batch_info", ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32),
+                                      ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_size_t)]),
+    ("srf_batch_project_ratio", ctypes.c_int, [_P, ctypes.c_int, _F, _P]),
+    ("srf_batch_backproject_update", ctypes.c_int, [_P, _F, _P]),
+    ("srf_batch_em_step", ctypes.c_int, [_P, ctypes.c_int, _F, _F, _P]),
     ("srf_plan_info", ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                                      ctypes.POINTER(ctypes.c_size_t)]),
     ("srf_plan_project", ctypes.c_int, [_P, _P, _P, _P]),
@@ -67,6 +86,7 @@ SIGNATURES = [
     ("srf_sss_rows_bytes", ctypes.c_size_t, [_I32, _I32]),
     ("srf_sss_prepare", ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _P, ctypes.c_size_t, _P]),
     ("srf_sss_sinogram", ctypes.c_int, [_P, _I32, _P, _I32, _P, _I32, _P, _F, _P, _P]),
+    ("srf_scatter_correct", ctypes.c_int, [_P, _P, _P, _P, _I64, _I64, _F, _P, _P, _P]),
     ("srf_cnn_conv", ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _F, _P, _P,
                                     _P]),
     ("srf_cnn_conv_wgrad", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P]),

@@ -1,0 +1,105 @@
+"""Autotuner for the batched reconstruction's launch schedule.
+
+Candidates (BatchConfig): plan work-item size (segments per workgroup pass),
+one workgroup per item or a persistent grid of 1-2 workgroups per CU, and the
+work-item order.  Each candidate is timed on one OSEM subset of the actual
+batch (plans built with its item size, a warm-up step, then the median of
+`reps` timed steps); the fastest wins, ties going to the earlier candidate.
+Decisions are cached in a JSON file under a signature of the device and the
+workload, so the same workload gets the same schedule without re-timing.
+
+Every candidate computes the same reconstruction: the launch mode and order do
+not change any sum, and the item size only changes how the backprojector's
+per-tile partials are grouped (float rounding; the per-scan images agree to
+about 1e-6).
+"""
+import json
+import math
+import os
+import time
+
+import torch
+
+from .batch import BatchConfig, BatchOsem
+
+CANDIDATES = tuple(BatchConfig(item_records=ir, persistent=p, workgroups_per_cu=w, order=o)
+                   for ir in (8192, 4096, 2048)
+                   for (p, w) in ((0, 1), (1, 1), (1, 2))
+                   for o in (1, 0))
+
+
+def signature(scans, use_tof, device_name=None):
+    """Workload key: device, TOF, scan count, log2 of the event total, and the
+    sorted image grids rounded to 32 voxels."""
+    if device_name is None:
+        device_name = torch.cuda.get_device_name(scans[0].lors.device) if torch.cuda.is_available() else "cpu"
+    ev = sum(int(s.lors.shape[0]) for s in scans)
+    grids = sorted(tuple(int(math.ceil(v / 32.0)) for v in s.shape) for s in scans)
+    return f"{device_name}|tof={int(bool(use_tof))}|scans={len(scans)}|ev2={int(round(math.log2(max(ev, 1))))}|" \
+           f"grids={grids}"
+
+
+def choose(timings, candidates=CANDIDATES):
+    """Index of the fastest candidate (first one on ties) from {index: seconds}."""
+    best = None
+    for i in range(len(candidates)):
+        if i in timings and (best is None or timings[i] < timings[best]):
+            best = i
+    if best is None:
+        raise ValueError("no timings")
+    return best
+
+
+class Autotuner:
+    def __init__(self, cache_path=None, candidates=CANDIDATES, reps=3):
+        self.cache_path = cache_path
+        self.candidates = tuple(candidates)
+        self.reps = int(reps)
+        self.cache = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.cache = json.load(f)
+
+    def _save(self):
+        if self.cache_path:
+            os.makedirs(os.path.dirname(os.path.abspath(self.cache_path)), exist_ok=True)
+            with open(self.cache_path, "w") as f:
+                json.dump(self.cache, f, indent=1, sort_keys=True)
+
+    def time_candidate(self, cand, scans, n_sub, tof_bin, tof_sigma2, use_tof, **kw):
+        b = BatchOsem(scans, n_sub, tof_bin, tof_sigma2, use_tof, config=cand, subsets=[0], **kw)
+        try:
+            b.step(0)
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(self.reps):
+                t = time.perf_counter()
+                b.step(0)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t)
+            return sorted(ts)[len(ts) // 2]
+        finally:
+            b.close()
+
+    def tune(self, scans, n_sub, tof_bin, tof_sigma2, use_tof=True, timer=None, log=None, **kw):
+        """BatchConfig for this workload (cached).  `timer(cand) -> seconds`
+        replaces the GPU timing (tests)."""
+        key = signature(scans, use_tof)
+        if key in self.cache:
+            return BatchConfig(**self.cache[key]["config"])
+        timings = {}
+        for i, cand in enumerate(self.candidates):
+            t = timer(cand) if timer is not None else self.time_candidate(cand, scans, n_sub, tof_bin, tof_sigma2,
+                                                                          use_tof, **kw)
+            timings[i] = float(t)
+            if log:
+                log(f"autotune {cand}: {1e3 * t:.2f} ms")
+        best = self.candidates[choose(timings, self.candidates)]
+        self.cache[key] = {"config": best.__dict__.copy(),
+                           "timings_ms": {str(self.candidates[i].__dict__): round(1e3 * v, 4)
+                                          for i, v in timings.items()}}
+        self._save()
+        return best
+
+
+__all__ = ["Autotuner", "CANDIDATES", "signature", "choose"]

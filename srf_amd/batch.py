@@ -1,0 +1,186 @@
+"""Batched TOF list-mode OSEM: many scans, one launch per direction per
+OSEM subset, HIP-graph-captured iterations.
+
+The reference reconstructs one scan per OsemFull call
+(srfnef/functions/osem_full.py:49-87, one TF op launch per projection); here
+every scan keeps OsemEngine's semantics (seeded permutation, contiguous
+subsets of n // n_sub events, q = y / (A x + eps), x <- x * A^T q / emap) and
+its result is bit-identical to running OsemEngine on that scan alone with the
+same plan item size, because the batched kernels run the same per-item device
+code and keep every reduction's order (srf_batch_* in include/srf_hip.h).
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import ops
+from ._lib import SrfBatchCfg, SrfBatchScan, check, lib, stream_handle
+from .functions import image_from_kernel, image_to_kernel, subset_permutation
+
+
+@dataclass(frozen=True)
+class BatchConfig:
+    """Launch schedule of a batch: plan work-item size (segments), persistent
+    grid or one workgroup per item, workgroups per CU, item order."""
+    item_records: int = 8192
+    persistent: int = 0
+    workgroups_per_cu: int = 1
+    order: int = 1
+
+    def cfg(self):
+        return SrfBatchCfg(int(self.persistent), int(self.workgroups_per_cu), int(self.order))
+
+
+@dataclass
+class Scan:
+    """One reconstruction: LORs (N, 7) on the GPU, image grid, sensitivity
+    image (x, y, z order), optional event values (default ones)."""
+    lors: torch.Tensor
+    shape: tuple
+    center: tuple
+    size: tuple
+    emap: object
+    values: Optional[torch.Tensor] = None
+    seed: int = 0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def geom(self):
+        return [float(v) for v in self.center] + [float(v) for v in self.size]
+
+
+class BatchOsem:
+    """OSEM state for a list of scans; one srf_batch per subset."""
+
+    def __init__(self, scans: List[Scan], n_sub, tof_bin, tof_sigma2, use_tof=True, config=BatchConfig(),
+                 ratio_kind=0, eps=1e-8, zero_guard=0.0, subsets=None):
+        self.scans, self.n_sub, self.config = scans, int(n_sub), config
+        self.ratio_kind, self.eps, self.zero_guard = int(ratio_kind), float(eps), float(zero_guard)
+        self.dev = scans[0].lors.device
+        self.x, self.emap, self.y, self.q, self.lors, self.perm, self.n_per_sub = [], [], [], [], [], [], []
+        for sc in scans:
+            n = sc.lors.shape[0]
+            perm = subset_permutation(n, sc.seed)
+            nps = n // self.n_sub
+            pidx = torch.as_tensor(perm[: nps * self.n_sub], device=self.dev, dtype=torch.long)
+            self.lors.append(sc.lors.index_select(0, pidx).contiguous())
+            vals = sc.values if sc.values is not None else torch.ones(n, dtype=torch.float32, device=self.dev)
+            self.y.append(vals.reshape(-1).to(self.dev, torch.float32).index_select(0, pidx).contiguous())
+            self.q.append(torch.empty(nps * self.n_sub, dtype=torch.float32, device=self.dev))
+            nx, ny, nz = [int(v) for v in sc.shape]
+            self.x.append(torch.ones((nz, ny, nx), dtype=torch.float32, device=self.dev))
+            self.emap.append(image_to_kernel(sc.emap).contiguous())
+            self.perm.append(perm)
+            self.n_per_sub.append(nps)
+        subsets = range(self.n_sub) if subsets is None else subsets
+        self.plans, self.batches = {}, {}
+        for k in subsets:
+            plans = []
+            for i, sc in enumerate(scans):
+                nps = self.n_per_sub[i]
+                plans.append(ops.Plan(self.lors[i][k * nps:(k + 1) * nps], sc.shape, sc.geom, use_tof, tof_bin,
+                                      tof_sigma2, item_records=config.item_records))
+            self.plans[k] = plans
+            self.batches[k] = self._make_batch(k, plans)
+        self.graph = None
+
+    def _make_batch(self, k, plans):
+        n = len(plans)
+        P = (ctypes.c_void_p * n)(*[p.handle for p in plans])
+        S = (SrfBatchScan * n)()
+        for i in range(n):
+            off = k * self.n_per_sub[i] * 4
+            S[i] = SrfBatchScan(self.x[i].data_ptr(), self.emap[i].data_ptr(), self.y[i].data_ptr() + off,
+                                self.q[i].data_ptr() + off)
+        h = ctypes.c_void_p()
+        cfg = self.config.cfg()
+        check(lib().srf_batch_create(P, S, n, ctypes.byref(cfg), ctypes.byref(h), stream_handle(self.dev)),
+              "srf_batch_create")
+        return h
+
+    def info(self, k=0):
+        ns, nw, gr, ne = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        nb = ctypes.c_size_t()
+        check(lib().srf_batch_info(self.batches[k], ctypes.byref(ns), ctypes.byref(nw), ctypes.byref(gr),
+                                   ctypes.byref(ne), ctypes.byref(nb)), "srf_batch_info")
+        return {"scans": ns.value, "work_items": nw.value, "grid": gr.value, "events": ne.value}
+
+    def step(self, k):
+        check(lib().srf_batch_em_step(self.batches[k], self.ratio_kind, self.eps, self.zero_guard,
+                                      stream_handle(self.dev)), "srf_batch_em_step")
+
+    def iteration(self):
+        for k in sorted(self.batches):
+            self.step(k)
+
+    def capture(self):
+        """Capture one full iteration (every subset) in a HIP graph."""
+        torch.cuda.synchronize(self.dev)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.iteration()
+        self.graph = g
+        return g
+
+    def run(self, n_iter, graph=False):
+        if graph:
+            if self.graph is None:
+                self.capture()
+            for _ in range(n_iter):
+                self.graph.replay()
+        else:
+            for _ in range(n_iter):
+                self.iteration()
+
+    def reset(self):
+        for x in self.x:
+            x.fill_(1.0)
+
+    def set_values(self, i, values):
+        """Replace scan i's event values (original event order) in place; the
+        batch reads them at every step, so a captured graph sees the change."""
+        pidx = torch.as_tensor(self.perm[i][: self.n_per_sub[i] * self.n_sub], device=self.dev, dtype=torch.long)
+        self.y[i].copy_(values.reshape(-1).to(self.dev, torch.float32).index_select(0, pidx))
+
+    def image(self, i):
+        """Scan i's estimate in (x, y, z) order."""
+        return image_from_kernel(self.x[i])
+
+    def close(self):
+        torch.cuda.synchronize(self.dev)
+        self.graph = None
+        for k, h in list(self.batches.items()):
+            if h:
+                lib().srf_batch_destroy(h)
+        self.batches = {}
+        for plans in self.plans.values():
+            for p in plans:
+                p.close()
+        self.plans = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def osem_single(scan: Scan, n_sub, n_iter, tof_bin, tof_sigma2, use_tof=True, item_records=8192, ratio_kind=0,
+                eps=1e-8, zero_guard=0.0):
+    """The same reconstruction for one scan with per-scan launches
+    (OsemEngine): the reference point for the batch's per-scan equality."""
+    from .functions import OsemEngine
+    from .data import TofConfig
+    n = scan.lors.shape[0]
+    vals = scan.values if scan.values is not None else torch.ones(n, dtype=torch.float32, device=scan.lors.device)
+    tof = TofConfig(tof_bin, tof_sigma2) if use_tof else None
+    eng = OsemEngine(scan.lors, vals, scan.emap, scan.shape, scan.geom, n_sub, subset_permutation(n, scan.seed),
+                     tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard, item_records=item_records)
+    eng.run(n_iter * n_sub)
+    return eng
+
+
+__all__ = ["BatchConfig", "Scan", "BatchOsem", "osem_single"]

@@ -90,6 +90,62 @@ __global__ __launch_bounds__(256) void k_sss_bins(const PairRow *__restrict__ ro
     out[b] = acc * scale;
 }
 
+// work[0] += sum_e sino[bins[e]], work[1] += sum_e y[e] (double, one atomic per block)
+__device__ __forceinline__ float bin_value(const float *__restrict__ sino, const int32_t *__restrict__ bins,
+                                           int64_t e, int64_t nbins)
+{
+    const int32_t b = bins[e];
+    return (b >= 0 && b < nbins) ? sino[b] : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_sums(const float *__restrict__ y, const int32_t *__restrict__ bins,
+                                                      const float *__restrict__ sino, int64_t nbins, int64_t n,
+                                                      double *__restrict__ work)
+{
+    __shared__ double rs[256], ry[256];
+    double a = 0.0, b = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        a += (double)bin_value(sino, bins, e, nbins);
+        b += (double)(y ? y[e] : 1.0f);
+    }
+    rs[threadIdx.x] = a;
+    ry[threadIdx.x] = b;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            rs[threadIdx.x] += rs[threadIdx.x + k];
+            ry[threadIdx.x] += ry[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(work, rs[0]);
+        atomicAdd(work + 1, ry[0]);
+    }
+}
+
+__global__ void k_zero2(double *__restrict__ work)
+{
+    if (threadIdx.x < 2) work[threadIdx.x] = 0.0;
+}
+
+// out[e] = (y[e] - sino[bins[e]] / S * Y * fraction) * w[e]   (S = 0: no subtraction)
+__global__ __launch_bounds__(256) void k_scatter_apply(const float *__restrict__ y, const float *__restrict__ w,
+                                                       const int32_t *__restrict__ bins,
+                                                       const float *__restrict__ sino, int64_t nbins, int64_t n,
+                                                       float fraction,
+                                                       const double *__restrict__ work, float *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double S = work[0];
+    const float c = S > 0.0 ? (float)(work[1] * (double)fraction / S) : 0.0f;
+    const float ye = y ? y[e] : 1.0f;
+    float v = ye - bin_value(sino, bins, e, nbins) * c;
+    if (w) v *= w[e];
+    out[e] = v;
+}
+
 }  // namespace srf
 
 using namespace srf;
@@ -123,6 +179,20 @@ int srf_sss_sinogram(const void *rows, int32_t n_det, const float *points, int32
     if (!rows || !points || !bins || !tables || !out) return SRF_ERR_INVALID_ARG;
     hipLaunchKernelGGL(k_sss_bins, dim3((unsigned)((n_bins + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        (const PairRow *)rows, points, n_det, n_points, bins, n_bins, tables, scale, out);
+    return launch_status();
+}
+
+int srf_scatter_correct(const float *y, const float *weight, const int32_t *bins, const float *sino, int64_t n_bins,
+                        int64_t n, float fraction, double *work, float *out, void *stream)
+{
+    if (n < 0 || n_bins < 1 || !bins || !sino || !work || !out) return SRF_ERR_INVALID_ARG;
+    if (n == 0) return SRF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, s, work);
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_scatter_sums, dim3(nb < 1024 ? nb : 1024), dim3(256), 0, s, y, bins, sino, n_bins, n, work);
+    hipLaunchKernelGGL(k_scatter_apply, dim3(nb), dim3(256), 0, s, y, weight, bins, sino, n_bins, n, fraction,
+                       (const double *)work, out);
     return launch_status();
 }
 

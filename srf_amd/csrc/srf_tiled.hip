@@ -25,6 +25,7 @@
 // cell lies in the tile, so every term of every event is applied exactly once.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -212,7 +213,8 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
                                                           const float *__restrict__ tofv, int tof_stride,
                                                           LaunchGeom G, TileCfg T, const int *__restrict__ ev_off,
                                                           unsigned *__restrict__ key_cursor,
-                                                          SegRec *__restrict__ recs, int *__restrict__ rec_ev)
+                                                          SegRec *__restrict__ recs, int *__restrict__ rec_ev,
+                                                          unsigned long long *__restrict__ rec_sk)
 {
     extern __shared__ unsigned hist[];
     for (int i = threadIdx.x; i < T.nkeys; i += BIN_THREADS) hist[i] = 0;
@@ -252,6 +254,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
             R.pad[0] = R.pad[1] = 0u;
             recs[pos] = R;
             rec_ev[pos] = (int)e;
+            rec_sk[pos] = ((unsigned long long)(tile * NCLASS + cls) << 32) | (unsigned)R.slot;
         });
     }
 }
@@ -437,14 +440,13 @@ __device__ __forceinline__ void walk_for(TileWalk &t, float *lds, int dummy, boo
 // LDS: the tile plus one zero word (gathers) / one scratch word per lane (scatters).
 constexpr int LDS_WORDS = TVOX + TILE_THREADS;
 
+// Projection of one work item: stage its tile in LDS, march every record,
+// one partial sum per record.
 template <bool TOF, bool DS, bool DZ>
-__global__ __launch_bounds__(TILE_THREADS) void k_fp_tiled(const SegRec *__restrict__ recs,
-                                                           const Item *__restrict__ items,
-                                                           const float *__restrict__ img, LaunchGeom G, TileCfg T,
-                                                           float *__restrict__ partial)
+__device__ __forceinline__ void fp_item(const SegRec *__restrict__ recs, const Item &it,
+                                        const float *__restrict__ img, const LaunchGeom &G, const TileCfg &T,
+                                        float *__restrict__ partial, float *tile)
 {
-    __shared__ float tile[LDS_WORDS];
-    const Item it = items[blockIdx.x];
     int x0, y0, z0;
     tile_origin(it.tile, T, x0, y0, z0);
     for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) {
@@ -460,6 +462,17 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fp_tiled(const SegRec *__restr
         march_tile<TOF, DS, DZ, M_GATHER>(R, G, (int)(R.ab_fp & 0xffffu), (int)(R.ab_fp >> 16), R.Wfp, t);
         partial[R.slot] = t.acc;
     }
+}
+
+template <bool TOF, bool DS, bool DZ>
+__global__ __launch_bounds__(TILE_THREADS) void k_fp_tiled(const SegRec *__restrict__ recs,
+                                                           const Item *__restrict__ items,
+                                                           const float *__restrict__ img, LaunchGeom G, TileCfg T,
+                                                           float *__restrict__ partial)
+{
+    __shared__ float tile[LDS_WORDS];
+    const Item it = items[blockIdx.x];
+    fp_item<TOF, DS, DZ>(recs, it, img, G, T, partial, tile);
 }
 
 __device__ __forceinline__ float event_value(const float *__restrict__ values, const float *__restrict__ y,
@@ -490,25 +503,22 @@ constexpr int FX_BITS = 49;
 constexpr int RECS_PER_THREAD = ITEM / TILE_THREADS;
 static_assert(ITEM % TILE_THREADS == 0, "whole records per thread");
 
+// Backprojection of one work item into the LDS tile fx (shared flags in
+// s_cmax / s_bad), written to out[0, TVOX) as float.
 template <bool TOF, bool DS, bool DZ>
-__global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restrict__ recs,
-                                                           const Item *__restrict__ items,
-                                                           const float *__restrict__ values,
-                                                           const float *__restrict__ y, const float *__restrict__ p,
-                                                           int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
-                                                           const int *__restrict__ rec_ev,
-                                                           float *__restrict__ tile_out)
+__device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const Item &it,
+                                        const float *__restrict__ values, const float *__restrict__ y,
+                                        const float *__restrict__ p, int kind, float eps, int adjoint,
+                                        const LaunchGeom &G, const TileCfg &T, const int *__restrict__ rec_ev,
+                                        float *__restrict__ out, unsigned long long *fx, unsigned *s_cmax,
+                                        int *s_bad)
 {
-    __shared__ unsigned long long fx[TVOX];
-    __shared__ unsigned s_cmax;
-    __shared__ int s_bad;
     float *fl = reinterpret_cast<float *>(fx);
-    const Item it = items[blockIdx.x];
     int x0, y0, z0;
     tile_origin(it.tile, T, x0, y0, z0);
     if (threadIdx.x == 0) {
-        s_cmax = 0u;
-        s_bad = 0;
+        *s_cmax = 0u;
+        *s_bad = 0;
     }
     for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) fx[i] = 0ull;
     const int end = it.start + it.count;
@@ -529,14 +539,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restr
     const float c = vmax * (adjoint ? it.wmax_fp : it.wmax_bp);
     const unsigned m = isfinite(c) ? __float_as_uint(c) : 0x7f7fffffu;
     __syncthreads();
-    if (m) atomicMax(&s_cmax, m);
-    if (bad) s_bad = 1;
+    if (m) atomicMax(s_cmax, m);
+    if (bad) *s_bad = 1;
     __syncthreads();
-    const bool fixed = s_bad == 0;
+    const bool fixed = *s_bad == 0;
     int shift = 0;
-    if (fixed && s_cmax) {
+    if (fixed && *s_cmax) {
         int ex = 0;
-        (void)frexpf(__uint_as_float(s_cmax), &ex);   // cmax < 2^ex
+        (void)frexpf(__uint_as_float(*s_cmax), &ex);   // cmax < 2^ex
         shift = FX_BITS - ex;
     }
 #pragma unroll
@@ -556,12 +566,28 @@ __global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restr
         else march_tile<TOF, DS, DZ, M_SCATTER>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
     }
     __syncthreads();
-    float *out = tile_out + (int64_t)blockIdx.x * TVOX;
     if (fixed) {
         for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = ldexpf((float)(long long)fx[i], -shift);
     } else {
         for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = fl[i];
     }
+}
+
+template <bool TOF, bool DS, bool DZ>
+__global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restrict__ recs,
+                                                           const Item *__restrict__ items,
+                                                           const float *__restrict__ values,
+                                                           const float *__restrict__ y, const float *__restrict__ p,
+                                                           int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
+                                                           const int *__restrict__ rec_ev,
+                                                           float *__restrict__ tile_out)
+{
+    __shared__ unsigned long long fx[TVOX];
+    __shared__ unsigned s_cmax;
+    __shared__ int s_bad;
+    const Item it = items[blockIdx.x];
+    bp_item<TOF, DS, DZ>(recs, it, values, y, p, kind, eps, adjoint, G, T, rec_ev,
+                         tile_out + (int64_t)blockIdx.x * TVOX, fx, &s_cmax, &s_bad);
 }
 
 template <bool TOF, bool DS, bool DZ>
@@ -662,6 +688,159 @@ __global__ void k_fill_u32(unsigned *__restrict__ d, unsigned v, int64_t n)
     if (i < n) d[i] = v;
 }
 
+__global__ void k_iota_i32(int *__restrict__ d, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) d[i] = (int)i;
+}
+
+// Records (and their event ids) permuted into the order `idx`.
+__global__ void k_permute_recs(const SegRec *__restrict__ recs, const int *__restrict__ rec_ev,
+                               const int *__restrict__ idx, int64_t n, SegRec *__restrict__ out,
+                               int *__restrict__ out_ev)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int j = idx[i];
+    out[i] = recs[j];
+    out_ev[i] = rec_ev[j];
+}
+
+// ---------------------------------------------------------------- batches --
+// A batch binds the plans of several scans (one OSEM subset each) and their
+// EM buffers.  Its work list holds every (scan, item) pair; the projector and
+// backprojector each run as ONE launch over it, either one workgroup per item
+// or a persistent grid (workgroups_per_cu x CUs) that pulls items from an
+// atomic queue.  Per item the work is exactly the single-plan kernels' (same
+// device functions), and the reductions keep their order, so every scan's
+// result is bit-identical to running its plan alone.
+
+struct ScanDesc {
+    LaunchGeom G;
+    TileCfg T;
+    const SegRec *recs;
+    const Item *items;
+    const int *rec_ev;
+    const int *tile_items;
+    const int *ev_off;
+    const uint8_t *ev_flag;
+    float *partial;
+    float *tile_buf;
+    float *x;             // image estimate (kernel layout), updated in place
+    const float *emap;    // sensitivity image (kernel layout)
+    const float *y;       // event values of the subset
+    float *q;             // ratio y / (A x + eps)
+    int64_t n;
+};
+
+struct Work {
+    int scan, item;
+};
+
+// Next work index of a workgroup: blockIdx.x first, then from the queue.
+__device__ __forceinline__ int next_work(int persistent, unsigned *counter, int *s_next)
+{
+    if (!persistent) return INT_MAX;
+    __syncthreads();   // the whole group is done with the LDS tile
+    if (threadIdx.x == 0) *s_next = (int)atomicAdd(counter, 1u) + (int)gridDim.x;
+    __syncthreads();
+    return *s_next;
+}
+
+template <bool TOF, bool DS, bool DZ>
+__global__ __launch_bounds__(TILE_THREADS) void k_fp_batch(const ScanDesc *__restrict__ desc,
+                                                           const Work *__restrict__ work, int nwork,
+                                                           unsigned *__restrict__ counter, int persistent)
+{
+    __shared__ float tile[LDS_WORDS];
+    __shared__ int s_next;
+    for (int w = blockIdx.x; w < nwork; w = next_work(persistent, counter, &s_next)) {
+        const Work wk = work[w];
+        const ScanDesc *d = desc + wk.scan;
+        const Item it = d->items[wk.item];
+        const LaunchGeom G = d->G;
+        const TileCfg T = d->T;
+        fp_item<TOF, DS, DZ>(d->recs, it, d->x, G, T, d->partial, tile);
+    }
+}
+
+template <bool TOF, bool DS, bool DZ>
+__global__ __launch_bounds__(TILE_THREADS) void k_bp_batch(const ScanDesc *__restrict__ desc,
+                                                           const Work *__restrict__ work, int nwork,
+                                                           unsigned *__restrict__ counter, int persistent)
+{
+    __shared__ unsigned long long fx[TVOX];
+    __shared__ unsigned s_cmax;
+    __shared__ int s_bad;
+    __shared__ int s_next;
+    for (int w = blockIdx.x; w < nwork; w = next_work(persistent, counter, &s_next)) {
+        const Work wk = work[w];
+        const ScanDesc *d = desc + wk.scan;
+        const Item it = d->items[wk.item];
+        const LaunchGeom G = d->G;
+        const TileCfg T = d->T;
+        bp_item<TOF, DS, DZ>(d->recs, it, d->q, nullptr, nullptr, 0, 0.0f, 0, G, T, d->rec_ev,
+                             d->tile_buf + (int64_t)wk.item * TVOX, fx, &s_cmax, &s_bad);
+    }
+}
+
+// index s with base[s] <= v < base[s + 1] (base non-decreasing, base[0] <= v < base[n])
+template <class T>
+__device__ __forceinline__ int find_segment(const T *__restrict__ base, int n, T v)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// q = ratio(y, sum of segment partials) for every event of every scan (the
+// order and arithmetic of k_seg_reduce).
+__global__ __launch_bounds__(256) void k_seg_reduce_batch(const ScanDesc *__restrict__ desc,
+                                                          const int64_t *__restrict__ ev_base, int nscan, int kind,
+                                                          float eps)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= ev_base[nscan]) return;
+    const int sc = find_segment(ev_base, nscan, e);
+    const ScanDesc *d = desc + sc;
+    const int64_t le = e - ev_base[sc];
+    float s = 0.0f;
+    for (int k = d->ev_off[le]; k < d->ev_off[le + 1]; ++k) s += d->partial[k];
+    if (d->ev_flag[le]) s = __builtin_nanf("");
+    const float den = s + eps;
+    float q = d->y[le] / den;
+    if (kind == SRF_RATIO_DIV_NO_NAN && den == 0.0f) q = 0.0f;
+    d->q[le] = q;
+}
+
+// Backprojected tiles summed in item order (k_tile_reduce), then the EM
+// update of srf_em_update: x = x * b / (emap == 0 && guard != 0 ? guard : emap).
+__global__ __launch_bounds__(256) void k_tile_update_batch(const ScanDesc *__restrict__ desc,
+                                                           const int *__restrict__ tile_base, int nscan, float guard)
+{
+    constexpr int BPT = TVOX / 256;   // blocks per tile
+    const int gt = blockIdx.x / BPT;
+    const int i = (blockIdx.x % BPT) * 256 + threadIdx.x;
+    if (gt >= tile_base[nscan]) return;
+    const int sc = find_segment(tile_base, nscan, gt);
+    const ScanDesc *d = desc + sc;
+    const int tile = gt - tile_base[sc];
+    int x0, y0, z0;
+    tile_origin(tile, d->T, x0, y0, z0);
+    const int gx = x0 + (i % TX), gy = y0 + (i / TX) % TY, gz = z0 + i / (TX * TY);
+    if (gx >= d->G.nx || gy >= d->G.ny || gz >= d->G.nz) return;
+    float s = 0.0f;
+    for (int k = d->tile_items[tile]; k < d->tile_items[tile + 1]; ++k) s += d->tile_buf[(int64_t)k * TVOX + i];
+    const int64_t v = gx + (int64_t)d->G.nx * (gy + (int64_t)d->G.ny * gz);
+    float e = d->emap[v];
+    if (guard != 0.0f && e == 0.0f) e = guard;
+    d->x[v] = d->x[v] * s / e;
+}
+
 }  // namespace srf
 
 using namespace srf;
@@ -738,8 +917,17 @@ extern "C" {
 int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const float *tof_values, int64_t tof_stride,
                     srf_grid_t grid, const srf_tof_t *tof, srf_plan **out, void *stream)
 {
+    return srf_plan_create_ex(lors, n, lor_stride, tof_values, tof_stride, grid, tof, ITEM, out, stream);
+}
+
+int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const float *tof_values,
+                       int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof, int32_t item_records,
+                       srf_plan **out, void *stream)
+{
     if (!out) return SRF_ERR_INVALID_ARG;
     *out = nullptr;
+    if (item_records < 256 || item_records > ITEM) return SRF_ERR_INVALID_ARG;
+    const unsigned item_max = (unsigned)item_records;
     if (n < 0 || (n > 0 && (!lors || lor_stride < 6))) return SRF_ERR_INVALID_ARG;
     if (tof && n > 0 && (!tof_values || tof_stride < 1)) return SRF_ERR_INVALID_ARG;
     if (n >= ((int64_t)1 << 31)) return SRF_ERR_UNSUPPORTED;
@@ -814,8 +1002,9 @@ int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const floa
         koff[k] = acc;
         const int tile = k / NCLASS;
         if (k % NCLASS == 0) tile_items[tile] = (int)items.size();
-        for (unsigned st = 0; st < kc[k]; st += ITEM)
-            items.push_back(Item{(int)(acc + st), (int)std::min<unsigned>(ITEM, kc[k] - st), tile, 0, 0.0f, 0.0f, {0.0f, 0.0f}});
+        for (unsigned st = 0; st < kc[k]; st += item_max)
+            items.push_back(Item{(int)(acc + st), (int)std::min<unsigned>(item_max, kc[k] - st), tile, 0, 0.0f, 0.0f,
+                                 {0.0f, 0.0f}});
         acc += kc[k];
     }
     tile_items[T.ntiles] = (int)items.size();
@@ -836,12 +1025,67 @@ int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const floa
                            s));
     SRF_HIP(hipMemcpyAsync(key_cursor, koff.data(), koff.size() * sizeof(unsigned), hipMemcpyHostToDevice, s));
     if (n > 0) {
+        unsigned long long *sk = nullptr, *sk2 = nullptr;
+        int *idx = nullptr, *idx2 = nullptr;
+        SegRec *recs2 = nullptr;
+        int *rec_ev2 = nullptr;
+        void *sort_tmp = nullptr;
+        auto cleanup_sort = [&]() {
+            (void)hipFree(sk);
+            (void)hipFree(sk2);
+            (void)hipFree(idx);
+            (void)hipFree(idx2);
+            (void)hipFree(sort_tmp);
+        };
+        auto fail = [&](int rc) {
+            cleanup_sort();
+            (void)hipFree(recs2);
+            (void)hipFree(rec_ev2);
+            cleanup();
+            return rc;
+        };
+        size_t tmp = 0;
+        if (dmalloc(&sk, (size_t)P->nseg, tmp) || dmalloc(&sk2, (size_t)P->nseg, tmp) ||
+            dmalloc(&idx, (size_t)P->nseg, tmp) || dmalloc(&idx2, (size_t)P->nseg, tmp) ||
+            dmalloc(&recs2, (size_t)P->nseg, tmp) || dmalloc(&rec_ev2, (size_t)P->nseg, tmp))
+            return fail(SRF_ERR_LAUNCH);
         if (P->tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs, P->rec_ev);
+                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs, P->rec_ev, sk);
         else
             hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs, P->rec_ev);
+                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs, P->rec_ev, sk);
+        // Deterministic record order: the emit places each record inside its key's
+        // range through atomic cursors, so the order within a key (and with it the
+        // split into work items and each item's fixed-point scale) would vary from
+        // run to run.  Sorting by (key, segment slot) fixes it: every plan of the
+        // same events is bit-identical.
+        if (P->nseg > 0) {
+            const unsigned nbs = (unsigned)((P->nseg + 255) / 256);
+            hipLaunchKernelGGL(k_iota_i32, dim3(nbs), dim3(256), 0, s, idx, P->nseg);
+            int key_bits = 1;
+            while ((1 << key_bits) < T.nkeys) ++key_bits;
+            size_t sort_bytes = 0;
+            if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, sk, sk2, idx, idx2, (int)P->nseg, 0,
+                                                   32 + key_bits, s) != hipSuccess ||
+                hipMalloc(&sort_tmp, sort_bytes ? sort_bytes : 1) != hipSuccess ||
+                hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, sk, sk2, idx, idx2, (int)P->nseg, 0,
+                                                   32 + key_bits, s) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(SRF_ERR_LAUNCH);
+            }
+            hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs, P->rec_ev, idx2, P->nseg, recs2,
+                               rec_ev2);
+            if (hipStreamSynchronize(s) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(SRF_ERR_LAUNCH);
+            }
+            std::swap(P->recs, recs2);
+            std::swap(P->rec_ev, rec_ev2);
+        }
+        cleanup_sort();
+        (void)hipFree(recs2);
+        (void)hipFree(rec_ev2);
     }
     if (P->nitems > 0) hipLaunchKernelGGL(k_item_wmax, dim3((unsigned)P->nitems), dim3(256), 0, s, P->recs, P->items);
     SRF_HIP(hipGetLastError());
@@ -960,3 +1204,220 @@ int srf_plan_backproject_ratio(srf_plan *P, const float *y, const float *p, srf_
 }
 
 }  // extern "C"
+
+// Scans whose plans use different kernel variants (TOF, voxel aspect) share a
+// batch: the work list is grouped by variant and each group is one launch.
+constexpr int NVARIANT = 8;   // tof << 2 | ds << 1 | dz
+
+struct srf_batch {
+    int nscan;
+    ScanDesc *desc;
+    Work *work;
+    int nwork;
+    int64_t *ev_base;
+    int *tile_base;
+    unsigned *counter;   // [2 g]: projector queue of group g, [2 g + 1]: backprojector queue
+    int64_t total_ev;
+    int total_tiles;
+    int persistent;
+    int grid;            // largest group's grid
+    int g_start[NVARIANT], g_count[NVARIANT], g_grid[NVARIANT];
+    size_t bytes;
+};
+
+static void batch_free(srf_batch *B)
+{
+    if (!B) return;
+    (void)hipFree(B->desc);
+    (void)hipFree(B->work);
+    (void)hipFree(B->ev_base);
+    (void)hipFree(B->tile_base);
+    (void)hipFree(B->counter);
+    delete B;
+}
+
+#define BATCH_DISPATCH(V, KERNEL, ...)                                                               \
+    do {                                                                                             \
+        switch (V) {                                                                                 \
+        case 0: KERNEL<false, false, false> __VA_ARGS__; break;                                      \
+        case 1: KERNEL<false, false, true> __VA_ARGS__; break;                                       \
+        case 2: KERNEL<false, true, false> __VA_ARGS__; break;                                       \
+        case 3: KERNEL<false, true, true> __VA_ARGS__; break;                                        \
+        case 4: KERNEL<true, false, false> __VA_ARGS__; break;                                       \
+        case 5: KERNEL<true, false, true> __VA_ARGS__; break;                                        \
+        case 6: KERNEL<true, true, false> __VA_ARGS__; break;                                        \
+        default: KERNEL<true, true, true> __VA_ARGS__; break;                                        \
+        }                                                                                            \
+    } while (0)
+
+static int plan_variant(const srf_plan *P) { return (P->tof ? 4 : 0) | (P->ds ? 2 : 0) | (P->dz ? 1 : 0); }
+
+extern "C" {
+
+int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int32_t n, const srf_batch_cfg_t *cfg,
+                     srf_batch **out, void *stream)
+{
+    if (!out) return SRF_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!plans || !scans || !cfg || n < 1) return SRF_ERR_INVALID_ARG;
+    if ((cfg->persistent != 0 && cfg->persistent != 1) || cfg->workgroups_per_cu < 1 ||
+        cfg->workgroups_per_cu > 16 || (cfg->order != 0 && cfg->order != 1))
+        return SRF_ERR_INVALID_ARG;
+    for (int i = 0; i < n; ++i) {
+        const srf_plan *P = plans[i];
+        if (!P || !scans[i].image || !scans[i].emap) return SRF_ERR_INVALID_ARG;
+        if (P->n > 0 && (!scans[i].y || !scans[i].q)) return SRF_ERR_INVALID_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    srf_batch *B = new srf_batch();
+    memset(B, 0, sizeof(*B));
+    B->nscan = n;
+    B->persistent = cfg->persistent;
+    auto cleanup = [&]() { batch_free(B); };
+    std::vector<ScanDesc> desc((size_t)n);
+    std::vector<int64_t> ev_base((size_t)n + 1, 0);
+    std::vector<int> tile_base((size_t)n + 1, 0);
+    std::vector<Work> work;
+    std::vector<int> cost, variant;
+    for (int i = 0; i < n; ++i) {
+        const srf_plan *P = plans[i];
+        ScanDesc &d = desc[(size_t)i];
+        d.G = P->G;
+        d.T = P->T;
+        d.recs = P->recs;
+        d.items = P->items;
+        d.rec_ev = P->rec_ev;
+        d.tile_items = P->tile_items;
+        d.ev_off = P->ev_off;
+        d.ev_flag = P->ev_flag;
+        d.partial = P->partial;
+        d.tile_buf = P->tile_buf;
+        d.x = scans[i].image;
+        d.emap = scans[i].emap;
+        d.y = scans[i].y;
+        d.q = scans[i].q;
+        d.n = P->n;
+        ev_base[(size_t)i + 1] = ev_base[(size_t)i] + P->n;
+        tile_base[(size_t)i + 1] = tile_base[(size_t)i] + P->T.ntiles;
+        if (P->nitems > 0) {
+            std::vector<Item> items((size_t)P->nitems);
+            SRF_HIP(hipMemcpyAsync(items.data(), P->items, items.size() * sizeof(Item), hipMemcpyDeviceToHost, s));
+            SRF_HIP(hipStreamSynchronize(s));
+            for (int k = 0; k < P->nitems; ++k) {
+                work.push_back(Work{i, k});
+                cost.push_back(items[(size_t)k].count);
+                variant.push_back(plan_variant(P));
+            }
+        }
+    }
+    {
+        // grouped by kernel variant; within a group in scan order, or longest
+        // items first (the tail of the launch is then made of short items)
+        std::vector<size_t> idx(work.size());
+        for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+            if (variant[a] != variant[b]) return variant[a] < variant[b];
+            return cfg->order == 1 && cost[a] > cost[b];
+        });
+        std::vector<Work> sorted(work.size());
+        for (size_t k = 0; k < idx.size(); ++k) sorted[k] = work[idx[k]];
+        for (int g = 0; g < NVARIANT; ++g) B->g_start[g] = B->g_count[g] = B->g_grid[g] = 0;
+        for (size_t k = 0; k < idx.size(); ++k) {
+            const int g = variant[idx[k]];
+            if (B->g_count[g] == 0) B->g_start[g] = (int)k;
+            ++B->g_count[g];
+        }
+        work.swap(sorted);
+    }
+    B->nwork = (int)work.size();
+    B->total_ev = ev_base[(size_t)n];
+    B->total_tiles = tile_base[(size_t)n];
+    int dev = 0, cus = 0;
+    SRF_HIP(hipGetDevice(&dev));
+    SRF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    B->grid = 0;
+    for (int g = 0; g < NVARIANT; ++g) {
+        const int c = B->g_count[g];
+        B->g_grid[g] = c == 0 ? 0 : (cfg->persistent ? std::max(1, std::min(c, cus * cfg->workgroups_per_cu)) : c);
+        B->grid = std::max(B->grid, B->g_grid[g]);
+    }
+    SRF_TRY(dmalloc(&B->desc, desc.size(), B->bytes));
+    SRF_TRY(dmalloc(&B->work, std::max<size_t>(1, work.size()), B->bytes));
+    SRF_TRY(dmalloc(&B->ev_base, ev_base.size(), B->bytes));
+    SRF_TRY(dmalloc(&B->tile_base, tile_base.size(), B->bytes));
+    SRF_TRY(dmalloc(&B->counter, 2 * NVARIANT, B->bytes));
+    SRF_HIP(hipMemcpyAsync(B->desc, desc.data(), desc.size() * sizeof(ScanDesc), hipMemcpyHostToDevice, s));
+    if (!work.empty())
+        SRF_HIP(hipMemcpyAsync(B->work, work.data(), work.size() * sizeof(Work), hipMemcpyHostToDevice, s));
+    SRF_HIP(hipMemcpyAsync(B->ev_base, ev_base.data(), ev_base.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    SRF_HIP(hipMemcpyAsync(B->tile_base, tile_base.data(), tile_base.size() * sizeof(int), hipMemcpyHostToDevice,
+                           s));
+    SRF_HIP(hipStreamSynchronize(s));
+    *out = B;
+    return SRF_OK;
+}
+
+int srf_batch_destroy(srf_batch *batch)
+{
+    batch_free(batch);
+    return SRF_OK;
+}
+
+int srf_batch_info(const srf_batch *B, int32_t *n_scans, int32_t *n_work, int32_t *grid, int64_t *n_events,
+                   size_t *device_bytes)
+{
+    if (!B) return SRF_ERR_INVALID_ARG;
+    if (n_scans) *n_scans = B->nscan;
+    if (n_work) *n_work = B->nwork;
+    if (grid) *grid = B->grid;
+    if (n_events) *n_events = B->total_ev;
+    if (device_bytes) *device_bytes = B->bytes;
+    return SRF_OK;
+}
+
+int srf_batch_project_ratio(srf_batch *B, srf_ratio_kind_t kind, float eps, void *stream)
+{
+    if (!B) return SRF_ERR_INVALID_ARG;
+    if (kind != SRF_RATIO_PLAIN && kind != SRF_RATIO_DIV_NO_NAN) return SRF_ERR_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (B->persistent && B->nwork > 0)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, B->counter, 0u, (int64_t)(2 * NVARIANT));
+    for (int g = 0; g < NVARIANT; ++g) {
+        if (B->g_count[g] == 0) continue;
+        BATCH_DISPATCH(g, k_fp_batch, <<<B->g_grid[g], TILE_THREADS, 0, s>>>(B->desc, B->work + B->g_start[g],
+                                                                             B->g_count[g], B->counter + 2 * g,
+                                                                             B->persistent));
+    }
+    if (B->total_ev > 0)
+        hipLaunchKernelGGL(k_seg_reduce_batch, dim3((unsigned)((B->total_ev + 255) / 256)), dim3(256), 0, s, B->desc,
+                           B->ev_base, B->nscan, (int)kind, eps);
+    return launch_status();
+}
+
+int srf_batch_backproject_update(srf_batch *B, float zero_guard, void *stream)
+{
+    if (!B) return SRF_ERR_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (B->persistent && B->nwork > 0)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, B->counter, 0u, (int64_t)(2 * NVARIANT));
+    for (int g = 0; g < NVARIANT; ++g) {
+        if (B->g_count[g] == 0) continue;
+        BATCH_DISPATCH(g, k_bp_batch, <<<B->g_grid[g], TILE_THREADS, 0, s>>>(B->desc, B->work + B->g_start[g],
+                                                                             B->g_count[g], B->counter + 2 * g + 1,
+                                                                             B->persistent));
+    }
+    if (B->total_tiles > 0)
+        hipLaunchKernelGGL(k_tile_update_batch, dim3((unsigned)B->total_tiles * (TVOX / 256)), dim3(256), 0, s,
+                           B->desc, B->tile_base, B->nscan, zero_guard);
+    return launch_status();
+}
+
+int srf_batch_em_step(srf_batch *B, srf_ratio_kind_t kind, float eps, float zero_guard, void *stream)
+{
+    int rc = srf_batch_project_ratio(B, kind, eps, stream);
+    if (rc) return rc;
+    return srf_batch_backproject_update(B, zero_guard, stream);
+}
+
+}  // extern "C"
+

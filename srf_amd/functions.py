@@ -137,7 +137,7 @@ class EmEngine:
     otherwise the one-event-per-lane kernels run on the raw LOR array."""
 
     def __init__(self, lors_data, values, emap_data, shape, geom, tof=None, ratio_kind=0, eps=0.0,
-                 zero_guard=0.0, tiled=True, plan=True):
+                 zero_guard=0.0, tiled=True, plan=True, item_records=8192):
         self.lors = to_gpu(lors_data)
         self.y = to_gpu(values).reshape(-1)
         self.shape = [int(v) for v in shape]
@@ -150,10 +150,12 @@ class EmEngine:
         self.eps = float(eps)
         self.zero_guard = float(zero_guard)
         self.tiled = bool(tiled)
+        self.item_records = int(item_records)
         self.plan = self.make_plan(self.lors) if (self.tiled and plan) else None
 
     def make_plan(self, lors):
-        return ops.Plan(lors, self.shape, self.geom, self.use_tof, self.tof_bin, self.tof_sigma2)
+        return ops.Plan(lors, self.shape, self.geom, self.use_tof, self.tof_bin, self.tof_sigma2,
+                        item_records=self.item_records)
 
     def forward(self, lors=None, plan=None):
         if plan is None and lors is None:
@@ -265,7 +267,7 @@ class OsemEngine(EmEngine):
     with ``tiled`` each subset is binned into its own plan at construction."""
 
     def __init__(self, lors_data, values, emap_data, shape, geom, n_sub, perm, tof=None, ratio_kind=0,
-                 eps=1e-8, zero_guard=0.0, tiled=True):
+                 eps=1e-8, zero_guard=0.0, tiled=True, item_records=8192):
         lors = to_gpu(lors_data)
         y = to_gpu(values).reshape(-1)
         n = lors.shape[0]
@@ -275,7 +277,7 @@ class OsemEngine(EmEngine):
                                dtype=torch.long)
         super().__init__(lors.index_select(0, pidx).contiguous(), y.index_select(0, pidx).contiguous(),
                          emap_data, shape, geom, tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard,
-                         tiled=tiled, plan=False)
+                         tiled=tiled, plan=False, item_records=item_records)
         del lors, y
         self.plans = [self.make_plan(self.subset(k)[0]) for k in range(self.n_sub)] if self.tiled else None
 

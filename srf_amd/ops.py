@@ -258,8 +258,9 @@ class Plan:
     round-off.  Device memory is owned by the plan and freed by close()/GC."""
 
     def __init__(self, lors: Tensor, shape, geom, use_tof: bool, tof_bin: float = 0.0, tof_sigma2: float = 0.0,
-                 tof_values: Optional[Tensor] = None):
+                 tof_values: Optional[Tensor] = None, item_records: int = 8192):
         n, stride, tp, ts = _lor_args(lors, tof_values, use_tof)
+        self.item_records = int(item_records)
         self.shape = [int(v) for v in shape]
         self.geom = [float(v) for v in geom]
         self.use_tof, self.tof_bin, self.tof_sigma2 = bool(use_tof), float(tof_bin), float(tof_sigma2)
@@ -267,10 +268,10 @@ class Plan:
         self.device = lors.device
         self._h = ctypes.c_void_p()
         with torch.cuda.device(lors.device):
-            rc = lib().srf_plan_create(lors.data_ptr(), n, stride, tp, ts, _grid(self.shape, self.geom),
-                                       _tof(use_tof, tof_bin, tof_sigma2), ctypes.byref(self._h),
-                                       stream_handle(lors.device))
-        check(rc, "srf_plan_create")
+            rc = lib().srf_plan_create_ex(lors.data_ptr(), n, stride, tp, ts, _grid(self.shape, self.geom),
+                                          _tof(use_tof, tof_bin, tof_sigma2), self.item_records,
+                                          ctypes.byref(self._h), stream_handle(lors.device))
+        check(rc, "srf_plan_create_ex")
         self._lib = lib()
 
     @property

@@ -290,11 +290,26 @@ class ScatterCorrect:
                                else listmode.lors.data).to(sino.device, torch.float32)
         y = torch.as_tensor(np.asarray(listmode.data) if not isinstance(listmode.data, torch.Tensor)
                             else listmode.data).to(sino.device, torch.float32).reshape(-1)
-        s_e = sino.reshape(-1)[g.bin_of_lors(lors)]
-        tot = s_e.sum()
-        corr = y - (s_e / tot * y.sum() * self.scatter_fraction if float(tot) > 0 else 0.0)
+        bins = g.bin_of_lors(lors).to(torch.int32).contiguous()
+        corr = scatter_correct(y, None, bins, sino, self.scatter_fraction)
         data = corr if isinstance(listmode.data, torch.Tensor) else corr.cpu().numpy()
         return Listmode(data, listmode.lors)
 
 
-__all__ = ["SssGeometry", "SingleScatterSimulation", "ScatterCorrect", "energy_tables", "sss_scale"]
+def scatter_correct(y, weight, bins, sino, fraction, out=None):
+    """(y - sino[bins] / sum(sino[bins]) * sum(y) * fraction) * weight per event
+    (scatter.py:51-53), by srf_scatter_correct.  y / weight None = ones."""
+    n = bins.numel()
+    dev = bins.device
+    out = torch.empty(n, dtype=torch.float32, device=dev) if out is None else out
+    work = torch.empty(2, dtype=torch.float64, device=dev)
+    sino = sino.reshape(-1).contiguous()
+    check(lib().srf_scatter_correct(y.data_ptr() if y is not None else None,
+                                    weight.data_ptr() if weight is not None else None, bins.data_ptr(),
+                                    sino.data_ptr(), sino.numel(), n, float(fraction), work.data_ptr(),
+                                    out.data_ptr(), stream_handle(dev)), "srf_scatter_correct")
+    return out
+
+
+__all__ = ["SssGeometry", "SingleScatterSimulation", "ScatterCorrect", "energy_tables", "sss_scale",
+           "scatter_correct"]

@@ -1,0 +1,93 @@
+"""The trained scatter surrogate (srf_amd/weights/scatter_unet_128x128x64.npz,
+made by `python -m srf_amd.train_surrogate`) against the SSS kernel on the 200
+held-out phantoms (seeds from HELD_OUT_SEED0, disjoint from training seeds
+0..1999), what the network gains over the estimate without it, and the SSS
+fallback."""
+import os
+import time
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "srf_amd", "weights", "scatter_unet_128x128x64.npz")
+
+
+@pytest.fixture(scope="module")
+def setup():
+    from srf_amd import train_surrogate as ts
+    from srf_amd.surrogate import UNet
+    net, meta = UNet.load(WEIGHTS, device="cuda")
+    g, center, size, sss, feat = ts.workload(torch.device("cuda", 0))
+    return ts, net, meta, g, center, size, sss, feat
+
+
+def test_heldout_error_within_two_percent(setup):
+    from srf_amd import phantom
+    from srf_amd.surrogate import relative_l2
+    ts, net, meta, g, center, size, sss, feat = setup
+    errs, naive = [], []
+    for k in range(200):
+        act, mu, _, _ = phantom.scatter_phantom(ts.SHAPE, ts.VOXEL, ts.HELD_OUT_SEED0 + k, device="cuda")
+        ref = sss(act, mu).reshape(-1)
+        x, m = feat(act, mu)
+        pred = net.forward(x, feat.dims) * (m / meta["kappa"])
+        errs.append(relative_l2(pred, ref))
+        # without the network: the attenuated emission projection e*exp(-mu) (feature 2),
+        # scaled by its least-squares factor against SSS -- the best that channel does alone
+        from srf_amd.surrogate import bf16_bits_to_float
+        f2 = bf16_bits_to_float(x[:, 2]).double()
+        c = float((f2 * ref.double()).sum() / (f2 * f2).sum())
+        naive.append(relative_l2(f2 * c, ref))
+        if (k + 1) % 50 == 0:
+            print(f"{k + 1}/200 mean {np.mean(errs):.4f} max {np.max(errs):.4f}", flush=True)
+    errs, naive = np.asarray(errs), np.asarray(naive)
+    print(f"held-out rel L2: mean {errs.mean():.4f} p95 {np.percentile(errs, 95):.4f} max {errs.max():.4f}; "
+          f"without the network (scaled attenuated emission): mean {naive.mean():.4f}", flush=True)
+    assert errs.max() <= 0.02
+    assert errs.mean() <= 0.2 * naive.mean()
+
+
+def test_surrogate_is_faster_than_sss(setup):
+    from srf_amd import phantom
+    from srf_amd.surrogate import ScatterSurrogate
+    ts, net, meta, g, center, size, sss, feat = setup
+    act, mu, _, _ = phantom.scatter_phantom(ts.SHAPE, ts.VOXEL, ts.HELD_OUT_SEED0 + 7, device="cuda")
+    sur = ScatterSurrogate(net, meta["kappa"], g, ts.SHAPE, center, size, check_rings=())
+
+    def t_of(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        ts_ = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts_.append(time.perf_counter() - t)
+        return float(np.median(ts_))
+
+    t_sur = t_of(lambda: sur.predict(act, mu), 10)
+    t_sss = t_of(lambda: sss(act, mu), 5)
+    print(f"surrogate {1e3 * t_sur:.2f} ms, SSS {1e3 * t_sss:.1f} ms: {t_sss / t_sur:.1f}x", flush=True)
+    assert t_sss >= 10.0 * t_sur
+
+
+def test_check_subset_and_fallback(setup):
+    from srf_amd import phantom
+    from srf_amd.surrogate import ScatterSurrogate
+    ts, net, meta, g, center, size, sss, feat = setup
+    act, mu, _, _ = phantom.scatter_phantom(ts.SHAPE, ts.VOXEL, ts.HELD_OUT_SEED0 + 3, device="cuda")
+    good = ScatterSurrogate(net, meta["kappa"], g, ts.SHAPE, center, size, check_rings=(5, 16, 26))
+    est, info = good.estimate(act, mu)
+    print("check", info, flush=True)
+    assert not info["fallback"] and info["check_err"] <= 0.02
+    assert torch.equal(est, good.predict(act, mu))
+    # a mis-scaled network fails the check, and the full SSS is returned instead
+    bad = ScatterSurrogate(net, 1.5 * meta["kappa"], g, ts.SHAPE, center, size, check_rings=(16,))
+    est, info = bad.estimate(act, mu)
+    print("check (mis-scaled)", info, flush=True)
+    assert info["fallback"] and info["check_err"] > 0.02
+    assert torch.equal(est, sss(act, mu))
