@@ -46,6 +46,14 @@ def parse():
                     help="one-event-per-lane kernels instead of the LDS-tiled plans")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
+    ap.add_argument("--no-scatter", action="store_true", help="skip the surrogate-vs-SSS scatter timing")
+    ap.add_argument("--scatter-phantoms", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scan reconstruction")
+    ap.add_argument("--batch-scans", type=int, default=32)
+    ap.add_argument("--batch-seed", type=int, default=5)
+    ap.add_argument("--batch-only", action="store_true", help="only the batched many-scan reconstruction")
+    ap.add_argument("--tof-window", type=float, default=26.0,
+                    help="plan TOF window (terms below 2^-w of the event's TOF peak are not binned; 0 = all terms)")
     return ap.parse_args()
 
 
@@ -73,6 +81,9 @@ def main():
     import numpy as np
     import torch
 
+    if args.batch_only:
+        print(json.dumps({"batch": batch_bench(args)}), flush=True)
+        return
     from srf_amd import ops, phantom
     from srf_amd.corrections import attenuation_factors
     from srf_amd.data import TofConfig, Umap
@@ -109,20 +120,28 @@ def main():
     tp = time.time()
     eng = OsemEngine(lors, y, emap, shape, geom, args.subsets, perm,
                      tof=TofConfig(tof_bin, tof_sigma2),
-                     ratio_kind=1, eps=1e-5, zero_guard=1e8, tiled=not args.event_kernels)
+                     ratio_kind=1, eps=1e-5, zero_guard=1e8, tiled=not args.event_kernels,
+                     tof_window=0.0 if args.event_kernels else args.tof_window)
     torch.cuda.synchronize()
     plan_s = time.time() - tp
     plan_info = eng.plans[0].info() if eng.plans else None
     log(f"engine ready in {plan_s:.1f}s; plan of subset 0: {plan_info}")
     del lors, y
-    # exact term counts T_e per event (same marcher as the kernels; GPU digest)
+    # term counts T_e per event: the reference's (the event kernels' digest) and,
+    # with a TOF window, the terms the plans hold (the plan's own digest)
     n_sub_ev = eng.n_per_sub
     l0, _ = eng.subset(0)
     _, cnt_fp = ops.lor_digest(l0, list(shape), geom, True, tof_bin, tof_sigma2, 1)
     _, cnt_bp = ops.lor_digest(l0, list(shape), geom, True, tof_bin, tof_sigma2, 0)
-    te_fp = float(cnt_fp.double().mean())
-    te_bp = float(cnt_bp.double().mean())
-    log(f"mean terms per event: fp {te_fp:.1f} bp {te_bp:.1f}")
+    te_ref_fp = float(cnt_fp.double().mean())
+    te_ref_bp = float(cnt_bp.double().mean())
+    te_fp, te_bp = te_ref_fp, te_ref_bp
+    if eng.plans:
+        _, c1 = eng.plans[0].digest(1)
+        _, c0 = eng.plans[0].digest(0)
+        te_fp, te_bp = float(c1.double().mean()), float(c0.double().mean())
+    del cnt_fp, cnt_bp
+    log(f"mean terms per event: fp {te_fp:.1f} bp {te_bp:.1f} (reference {te_ref_fp:.1f} / {te_ref_bp:.1f})")
 
     # per-kernel timing with HIP events on the launch stream (torch's current stream)
     stream = torch.cuda.current_stream()
@@ -163,6 +182,7 @@ def main():
     torch.cuda.synchronize()
     t = time.perf_counter() - t_start
     n_used = eng.n_per_sub * eng.n_sub
+    n_sub = eng.n_sub
     value = n_used * args.steps / t
     fp_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
     bp_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
@@ -187,23 +207,139 @@ def main():
             "frac": dom[2] / HBM_PEAK, "traffic": traffic, "kernel": dom[0], "mean_launch_ms": dom[1],
             "algorithmic_bytes_per_launch": dom[3],
             "other": {"project_ms": fp_ms, "backproject_ms": bp_ms, "project_GBps": fp_bw / 1e9,
-                      "backproject_GBps": bp_bw / 1e9, "terms_per_event_fp": te_fp, "terms_per_event_bp": te_bp}}
+                      "backproject_GBps": bp_bw / 1e9, "terms_per_event_fp": te_fp, "terms_per_event_bp": te_bp,
+                      "terms_per_event_reference_fp": te_ref_fp, "terms_per_event_reference_bp": te_ref_bp}}
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(eng, shape, center, size, tof_bin, tof_sigma2, args.cpu_sample)
+    for p in eng.plans or []:
+        p.close()
+    del eng
+    torch.cuda.empty_cache()
+    scatter = None if args.no_scatter else scatter_bench(args)
+    batch = None if args.no_batch else batch_bench(args)
 
     out = {"metric": METRIC, "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": t / args.steps * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "TOF list-mode OSEM, 256x256x128 x 2 mm, 5e7 events, 10 subsets, "
                                   "R 424 mm, axial 300 mm, TOF FWHM 60 mm (400 ps), water mu-map",
-                      "events": n_used, "subsets": eng.n_sub, "image": list(shape), "voxel_mm": voxel,
+                      "events": n_used, "subsets": n_sub, "image": list(shape), "voxel_mm": voxel,
                       "tof_bin_mm": tof_bin, "tof_sigma2_mm2": tof_sigma2, "seed": args.seed,
                       "kernels": "event-per-lane" if args.event_kernels else "lds-tiled plans",
+                      "tof_window": 0.0 if args.event_kernels else args.tof_window,
                       "setup_binning_s": plan_s, "plan_subset0": plan_info},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu, "scatter": scatter, "batch": batch}
     print(json.dumps(out), flush=True)
+
+
+WEIGHTS = os.path.join(ROOT, "srf_amd", "weights", "scatter_unet_128x128x64.npz")
+
+
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    r = fn()
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t
+
+
+def scatter_bench(args):
+    """Scatter estimate per (emission, mu) pair on the surrogate workload
+    (128x128x64, 4 mm voxels, 8 mm scatter grid, 32x64x64 sinogram): SSS kernel
+    vs the CNN surrogate on held-out phantoms, relative L2 of the surrogate
+    against SSS; plus the surrogate with its one-ring SSS check."""
+    import numpy as np
+    import torch
+    from srf_amd import phantom, train_surrogate as ts
+    from srf_amd.surrogate import ScatterSurrogate, UNet, relative_l2
+    dev = torch.device("cuda", 0)
+    net, meta = UNet.load(WEIGHTS, device=dev)
+    g, center, size, sss, feat = ts.workload(dev)
+    sur = ScatterSurrogate(net, meta["kappa"], g, ts.SHAPE, center, size, check_rings=())
+    chk = ScatterSurrogate(net, meta["kappa"], g, ts.SHAPE, center, size, check_rings=(16,))
+    errs, t_sss, t_sur, t_chk, fallbacks = [], [], [], [], 0
+    for k in range(args.scatter_phantoms + 1):
+        act, mu, _, _ = phantom.scatter_phantom(ts.SHAPE, ts.VOXEL, ts.HELD_OUT_SEED0 + k, device=dev)
+        ref, a = _timed(lambda: sss(act, mu))
+        pred, b = _timed(lambda: sur.predict(act, mu))
+        (est, info), c = _timed(lambda: chk.estimate(act, mu))
+        if k == 0:
+            continue          # first call: allocations and code-object loading
+        errs.append(relative_l2(pred, ref))
+        t_sss.append(a)
+        t_sur.append(b)
+        t_chk.append(c)
+        fallbacks += int(info["fallback"])
+    log(f"scatter: SSS {1e3 * np.median(t_sss):.1f} ms, surrogate {1e3 * np.median(t_sur):.2f} ms, "
+        f"rel L2 mean {np.mean(errs):.4f} max {np.max(errs):.4f}")
+    return {"workload": "emission + mu 128x128x64 x 4 mm, 8 mm scatter grid, 400-650 keV, sinogram 32 rings x "
+                        "64 angles x 64 radial, 4096 virtual detectors",
+            "phantoms": len(errs), "sss_ms": 1e3 * float(np.median(t_sss)),
+            "surrogate_ms": 1e3 * float(np.median(t_sur)),
+            "surrogate_with_check_ms": 1e3 * float(np.median(t_chk)),
+            "speedup": float(np.median(t_sss) / np.median(t_sur)),
+            "speedup_with_check": float(np.median(t_sss) / np.median(t_chk)),
+            "rel_l2_mean": float(np.mean(errs)), "rel_l2_max": float(np.max(errs)),
+            "check_fallbacks": fallbacks, "heldout_200_from_training": {
+                k: meta[k] for k in ("heldout_mean", "heldout_p95", "heldout_max") if k in meta}}
+
+
+def batch_bench(args):
+    """Batched reconstructions: 32 scans (events log-uniform 1e5-2e7, in-plane
+    96-256, 64-128 slices, 2-4 mm voxels), TOF OSEM 8 subsets x 4 iterations with
+    scatter re-estimated before iterations 2-4.  Plain method: per-scan launches
+    + SSS.  Batched method: one BatchOsem (autotuned schedule, HIP graph) + the
+    CNN surrogate with its SSS check.  The batched images are compared with
+    per-scan runs using the same surrogate scatter."""
+    import tempfile
+
+    import numpy as np
+    import torch
+    from srf_amd import phantom
+    from srf_amd.autotune import Autotuner
+    from srf_amd.pipeline import (FWHM, TOF_BIN, TOF_WINDOW, ScatterEstimator, make_scans, reconstruct_batched,
+                                  reconstruct_plain)
+    from srf_amd.surrogate import UNet
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    scans = make_scans(n_scans=args.batch_scans, seed=args.batch_seed, device=dev)
+    n_ev = sum(int(s.lors.shape[0]) for s in scans)
+    log(f"batch workload: {len(scans)} scans, {n_ev} events, setup {time.time() - t0:.1f}s")
+    net, meta = UNet.load(WEIGHTS, device=dev)
+    kappa = meta["kappa"]
+    cache = os.environ.get("SRF_AUTOTUNE_CACHE", os.path.join(tempfile.gettempdir(), "srf_amd_autotune.json"))
+    tuner = Autotuner(cache, reps=3)
+    s2 = phantom.tof_sigma2_from_fwhm(FWHM)
+    (cfg, t_tune) = _timed(lambda: tuner.tune(scans, 8, TOF_BIN, s2, True, tof_window=TOF_WINDOW))
+    log(f"autotuned {cfg} in {t_tune:.1f}s")
+    imgs_plain, t_plain = reconstruct_plain(scans, 8, 4, ScatterEstimator(net=net, kappa=kappa), scatter="sss",
+                                            log=log)
+    torch.cuda.empty_cache()
+    est_b = ScatterEstimator(net=net, kappa=kappa)
+    imgs_b, t_b, info = reconstruct_batched(scans, 8, 4, est_b, config=cfg, graph=True, log=log)
+    torch.cuda.empty_cache()
+    imgs_p, t_p = reconstruct_plain(scans, 8, 4, ScatterEstimator(net=net, kappa=kappa), scatter="surrogate",
+                                    item_records=cfg.item_records, log=log)
+    eq = [float((a - b).norm() / b.norm()) for a, b in zip(imgs_b, imgs_p)]
+    bitwise = all(bool(torch.equal(a, b)) for a, b in zip(imgs_b, imgs_p))
+    vs_sss = [float((a - b).norm() / b.norm()) for a, b in zip(imgs_b, imgs_plain)]
+    checks = [e for _, e in est_b.checks]
+    log("surrogate checks (scan, rel L2 on the check ring): " +
+        ", ".join(f"{i}:{e:.3f}" for i, e in est_b.checks))
+    return {"workload": "32 scans, events log-uniform 1e5-2e7, in-plane {96,128,192,256}, 64-128 slices, "
+                        "2-4 mm voxels, random ellipsoid phantoms, TOF FWHM 60 mm, OSEM 8 subsets x 4 "
+                        "iterations, scatter before iterations 2-4",
+            "scans": len(scans), "events": n_ev,
+            "plain_per_scan_sss_s": t_plain, "batched_graph_surrogate_s": t_b,
+            "speedup": t_plain / t_b, "per_scan_surrogate_s": t_p,
+            "image_rel_l2_batched_vs_per_scan_max": max(eq), "image_bitwise_equal": bitwise,
+            "image_rel_l2_surrogate_vs_sss_scatter_mean": float(np.mean(vs_sss)),
+            "check_err_max": max(checks) if checks else None,
+            "fallback_scans": sum(1 for v in est_b.fallback.values() if v),
+            "autotuned": cfg.__dict__, "autotune_s": t_tune, "batch_info": info}
 
 
 def cpu_baseline(eng, shape, center, size, tof_bin, tof_sigma2, sample):

@@ -148,13 +148,24 @@ typedef struct srf_plan srf_plan;
 int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride,
                     const float *tof_values, int64_t tof_stride,
                     srf_grid_t grid, const srf_tof_t *tof, srf_plan **plan, void *stream);
-/* srf_plan_create with the largest work item set to item_records segments
- * (256..8192; srf_plan_create uses 8192).  Smaller items spread a crowded
- * tile over more workgroups (more tile partials to sum); the batch autotuner
- * chooses it. */
+/* Plan options (srf_plan_create_ex; NULL = the srf_plan_create defaults). */
+typedef struct {
+    /* Largest work item in segments, 256..8192 (0 = 8192).  Smaller items
+     * spread a crowded tile over more workgroups (more tile partials to sum);
+     * the batch autotuner chooses it. */
+    int32_t item_records;
+    /* TOF window: 0 keeps every term, as the reference (siddon_tof.cu.cc
+     * weights every slab, :55-59).  w > 0 bins only the slabs whose TOF
+     * Gaussian factor is at least 2^-w of its peak (plus one slab each side):
+     * w = 26 drops < 2e-9 of an event's TOF weight, far below float32
+     * resolution, and about a third of the terms at 60 mm FWHM.  Events
+     * whose window misses the image project to 0. */
+    float tof_window;
+} srf_plan_opts_t;
+
 int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride,
                        const float *tof_values, int64_t tof_stride,
-                       srf_grid_t grid, const srf_tof_t *tof, int32_t item_records,
+                       srf_grid_t grid, const srf_tof_t *tof, const srf_plan_opts_t *opts,
                        srf_plan **plan, void *stream);
 int srf_plan_destroy(srf_plan *plan);
 int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments,

@@ -31,6 +31,10 @@ class SrfTof(ctypes.Structure):
     _fields_ = [("tof_bin", ctypes.c_float), ("tof_sigma2", ctypes.c_float)]
 
 
+class SrfPlanOpts(ctypes.Structure):
+    _fields_ = [("item_records", ctypes.c_int32), ("tof_window", ctypes.c_float)]
+
+
 class SrfBatchScan(ctypes.Structure):
     _fields_ = [("image", ctypes.c_void_p), ("emap", ctypes.c_void_p), ("y", ctypes.c_void_p),
                 ("q", ctypes.c_void_p)]
@@ -64,8 +68,8 @@ SIGNATURES = [
     ("srf_image_to_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
     ("srf_image_from_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
     ("srf_plan_create", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(_P), _P]),
-    ("srf_plan_create_ex", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, _I32, ctypes.POINTER(_P),
-                                          _P]),
+    ("srf_plan_create_ex", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(SrfPlanOpts),
+                                          ctypes.POINTER(_P), _P]),
     ("srf_plan_destroy", ctypes.c_int, [_P]),
     ("srf_batch_create", ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(SrfBatchScan), _I32,
                                         ctypes.POINTER(SrfBatchCfg), ctypes.POINTER(_P), _P]),

@@ -66,34 +66,45 @@ class Autotuner:
             with open(self.cache_path, "w") as f:
                 json.dump(self.cache, f, indent=1, sort_keys=True)
 
-    def time_candidate(self, cand, scans, n_sub, tof_bin, tof_sigma2, use_tof, **kw):
-        b = BatchOsem(scans, n_sub, tof_bin, tof_sigma2, use_tof, config=cand, subsets=[0], **kw)
-        try:
+    def _time_step(self, b):
+        b.step(0)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(self.reps):
+            t = time.perf_counter()
             b.step(0)
             torch.cuda.synchronize()
-            ts = []
-            for _ in range(self.reps):
-                t = time.perf_counter()
-                b.step(0)
-                torch.cuda.synchronize()
-                ts.append(time.perf_counter() - t)
-            return sorted(ts)[len(ts) // 2]
-        finally:
-            b.close()
+            ts.append(time.perf_counter() - t)
+        return sorted(ts)[len(ts) // 2]
+
+    def _gpu_timings(self, scans, n_sub, tof_bin, tof_sigma2, use_tof, log=None, **kw):
+        """Median step time of every candidate on subset 0; the plans of one
+        item size are built once and every launch schedule is bound to them."""
+        timings = {}
+        for ir in dict.fromkeys(c.item_records for c in self.candidates):
+            idx = [i for i, c in enumerate(self.candidates) if c.item_records == ir]
+            b = BatchOsem(scans, n_sub, tof_bin, tof_sigma2, use_tof, config=self.candidates[idx[0]], subsets=[0],
+                          **kw)
+            try:
+                for i in idx:
+                    b.set_schedule(self.candidates[i])
+                    timings[i] = float(self._time_step(b))
+                    if log:
+                        log(f"autotune {self.candidates[i]}: {1e3 * timings[i]:.2f} ms")
+            finally:
+                b.close()
+        return timings
 
     def tune(self, scans, n_sub, tof_bin, tof_sigma2, use_tof=True, timer=None, log=None, **kw):
         """BatchConfig for this workload (cached).  `timer(cand) -> seconds`
         replaces the GPU timing (tests)."""
-        key = signature(scans, use_tof)
+        key = signature(scans, use_tof) + "".join(f"|{k}={kw[k]}" for k in sorted(kw))
         if key in self.cache:
             return BatchConfig(**self.cache[key]["config"])
-        timings = {}
-        for i, cand in enumerate(self.candidates):
-            t = timer(cand) if timer is not None else self.time_candidate(cand, scans, n_sub, tof_bin, tof_sigma2,
-                                                                          use_tof, **kw)
-            timings[i] = float(t)
-            if log:
-                log(f"autotune {cand}: {1e3 * t:.2f} ms")
+        if timer is not None:
+            timings = {i: float(timer(c)) for i, c in enumerate(self.candidates)}
+        else:
+            timings = self._gpu_timings(scans, n_sub, tof_bin, tof_sigma2, use_tof, log=log, **kw)
         best = self.candidates[choose(timings, self.candidates)]
         self.cache[key] = {"config": best.__dict__.copy(),
                            "timings_ms": {str(self.candidates[i].__dict__): round(1e3 * v, 4)

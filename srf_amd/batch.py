@@ -18,7 +18,14 @@ import torch
 
 from . import ops
 from ._lib import SrfBatchCfg, SrfBatchScan, check, lib, stream_handle
-from .functions import image_from_kernel, image_to_kernel, subset_permutation
+from .functions import image_from_kernel, image_to_kernel
+
+
+def scan_permutation(n, seed, device):
+    """Seeded event permutation generated on the GPU (torch.randperm): the
+    subset order of a scan in both the batched and the per-scan runs."""
+    g = torch.Generator(device=device).manual_seed(int(seed))
+    return torch.randperm(int(n), generator=g, device=device)
 
 
 @dataclass(frozen=True)
@@ -56,16 +63,16 @@ class BatchOsem:
     """OSEM state for a list of scans; one srf_batch per subset."""
 
     def __init__(self, scans: List[Scan], n_sub, tof_bin, tof_sigma2, use_tof=True, config=BatchConfig(),
-                 ratio_kind=0, eps=1e-8, zero_guard=0.0, subsets=None):
+                 ratio_kind=0, eps=1e-8, zero_guard=0.0, subsets=None, tof_window=0.0):
         self.scans, self.n_sub, self.config = scans, int(n_sub), config
         self.ratio_kind, self.eps, self.zero_guard = int(ratio_kind), float(eps), float(zero_guard)
         self.dev = scans[0].lors.device
         self.x, self.emap, self.y, self.q, self.lors, self.perm, self.n_per_sub = [], [], [], [], [], [], []
         for sc in scans:
             n = sc.lors.shape[0]
-            perm = subset_permutation(n, sc.seed)
+            perm = scan_permutation(n, sc.seed, self.dev)
             nps = n // self.n_sub
-            pidx = torch.as_tensor(perm[: nps * self.n_sub], device=self.dev, dtype=torch.long)
+            pidx = perm[: nps * self.n_sub]
             self.lors.append(sc.lors.index_select(0, pidx).contiguous())
             vals = sc.values if sc.values is not None else torch.ones(n, dtype=torch.float32, device=self.dev)
             self.y.append(vals.reshape(-1).to(self.dev, torch.float32).index_select(0, pidx).contiguous())
@@ -82,7 +89,7 @@ class BatchOsem:
             for i, sc in enumerate(scans):
                 nps = self.n_per_sub[i]
                 plans.append(ops.Plan(self.lors[i][k * nps:(k + 1) * nps], sc.shape, sc.geom, use_tof, tof_bin,
-                                      tof_sigma2, item_records=config.item_records))
+                                      tof_sigma2, item_records=config.item_records, tof_window=tof_window))
             self.plans[k] = plans
             self.batches[k] = self._make_batch(k, plans)
         self.graph = None
@@ -100,6 +107,19 @@ class BatchOsem:
         check(lib().srf_batch_create(P, S, n, ctypes.byref(cfg), ctypes.byref(h), stream_handle(self.dev)),
               "srf_batch_create")
         return h
+
+    def set_schedule(self, config):
+        """Rebind the launch schedule (persistent / workgroups per CU / order)
+        on the same plans; the item size is fixed by the plans."""
+        if config.item_records != self.config.item_records:
+            raise ValueError("item_records is fixed when the plans are built")
+        torch.cuda.synchronize(self.dev)
+        self.graph = None
+        for h in self.batches.values():
+            if h:
+                lib().srf_batch_destroy(h)
+        self.config = config
+        self.batches = {k: self._make_batch(k, self.plans[k]) for k in self.plans}
 
     def info(self, k=0):
         ns, nw, gr, ne = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
@@ -142,7 +162,7 @@ class BatchOsem:
     def set_values(self, i, values):
         """Replace scan i's event values (original event order) in place; the
         batch reads them at every step, so a captured graph sees the change."""
-        pidx = torch.as_tensor(self.perm[i][: self.n_per_sub[i] * self.n_sub], device=self.dev, dtype=torch.long)
+        pidx = self.perm[i][: self.n_per_sub[i] * self.n_sub]
         self.y[i].copy_(values.reshape(-1).to(self.dev, torch.float32).index_select(0, pidx))
 
     def image(self, i):
@@ -169,7 +189,7 @@ class BatchOsem:
 
 
 def osem_single(scan: Scan, n_sub, n_iter, tof_bin, tof_sigma2, use_tof=True, item_records=8192, ratio_kind=0,
-                eps=1e-8, zero_guard=0.0):
+                eps=1e-8, zero_guard=0.0, tof_window=0.0):
     """The same reconstruction for one scan with per-scan launches
     (OsemEngine): the reference point for the batch's per-scan equality."""
     from .functions import OsemEngine
@@ -177,10 +197,11 @@ def osem_single(scan: Scan, n_sub, n_iter, tof_bin, tof_sigma2, use_tof=True, it
     n = scan.lors.shape[0]
     vals = scan.values if scan.values is not None else torch.ones(n, dtype=torch.float32, device=scan.lors.device)
     tof = TofConfig(tof_bin, tof_sigma2) if use_tof else None
-    eng = OsemEngine(scan.lors, vals, scan.emap, scan.shape, scan.geom, n_sub, subset_permutation(n, scan.seed),
-                     tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard, item_records=item_records)
+    eng = OsemEngine(scan.lors, vals, scan.emap, scan.shape, scan.geom, n_sub,
+                     scan_permutation(n, scan.seed, scan.lors.device), tof=tof, ratio_kind=ratio_kind, eps=eps,
+                     zero_guard=zero_guard, item_records=item_records, tof_window=tof_window)
     eng.run(n_iter * n_sub)
     return eng
 
 
-__all__ = ["BatchConfig", "Scan", "BatchOsem", "osem_single"]
+__all__ = ["BatchConfig", "Scan", "BatchOsem", "osem_single", "scan_permutation"]

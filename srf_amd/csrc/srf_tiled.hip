@@ -127,9 +127,31 @@ __device__ __forceinline__ void for_each_segment(const Branch &B, const RaySetup
 // ties (|xd| == |yd|) march different branches in the projector (>=,
 // siddon.cu.cc:46) and the backprojector (>, :372): both are binned, each
 // marked for its own direction.
+template <bool TOF>
+__device__ __forceinline__ void tof_coeffs_d(const Branch &B, const RaySetup &r, const LaunchGeom &G, double &tA,
+                                             double &tB);
+
+// TOF window (twin > 0): keep the slabs whose Gaussian factor
+// exp2(-(pp1 tA + tB)^2) is at least 2^-twin of its peak, plus one slab of
+// margin on each side for the rounding of the float coefficients.
+template <bool TOF>
+__device__ __forceinline__ void tof_window(const Branch &B, const RaySetup &r, const LaunchGeom &G, float twin,
+                                           int &lo, int &hi)
+{
+    if (!TOF || !(twin > 0.0f)) return;
+    double tA, tB;
+    tof_coeffs_d<TOF>(B, r, G, tA, tB);
+    if (!(fabs(tA) > 0.0) || !isfinite(tA) || !isfinite(tB)) return;
+    const double D = sqrt((double)twin);
+    const double u1 = (-D - tB) / tA, u2 = (D - tB) / tA;
+    const double np2 = B.np2;
+    lo = max(lo, dbl_to_int(ceil(fmin(u1, u2) + np2)) - 1);
+    hi = min(hi, dbl_to_int(floor(fmax(u1, u2) + np2)) + 2);
+}
+
 template <bool TOF, class F>
 __device__ __forceinline__ void for_each_event_segment(const RaySetup &r, const LaunchGeom &G, const TileCfg &T,
-                                                       F &&f)
+                                                       float twin, F &&f)
 {
     if (!r.live) return;
     const bool fp_live = TOF || !r.q2;
@@ -155,6 +177,8 @@ __device__ __forceinline__ void for_each_event_segment(const RaySetup &r, const 
         ctx.fa = lo;
         ctx.fb = hi;
         if (ctx.fp_use) fp_cut_range<TOF>(B, lo, hi, ctx.fa, ctx.fb);
+        tof_window<TOF>(B, r, G, twin, lo, hi);
+        tof_window<TOF>(B, r, G, twin, ctx.fa, ctx.fb);
         int elo = ctx.bp_use ? lo : ctx.fa, ehi = ctx.bp_use ? hi : ctx.fb;
         for_each_segment(B, r, G, T, elo, ehi, [&](int tile, int cls, int a, int b) { f(tile, cls, a, b, B, ctx); });
     }
@@ -169,22 +193,33 @@ __device__ __forceinline__ uint32_t pack_ab(int a, int b) { return (uint32_t)a |
 //   exp(-d^2 / (2 tse)) = exp2(-(d * sqrt(log2(e) / (2 tse)))^2)
 // Coefficients in double, rounded once.
 template <bool TOF>
-__device__ __forceinline__ void tof_coeffs(const Branch &B, const RaySetup &r, const LaunchGeom &G, float &tA,
-                                           float &tB)
+__device__ __forceinline__ void tof_coeffs_d(const Branch &B, const RaySetup &r, const LaunchGeom &G, double &tA,
+                                             double &tB)
 {
-    tA = tB = 0.0f;
+    tA = tB = 0.0;
     if (!TOF) return;
     const double off = (double)B.np2 - (double)(B.np / 2) + 0.5;
     const double s = sqrt(0.5 * 1.4426950408889634 / (double)G.tse);
     const double k = (double)r.L / ((double)B.p2 - (double)B.p1);
-    tA = (float)((double)B.pd_tof * k * s);
-    tB = (float)(((off * (double)B.pd_tof - (double)B.p1) * k - (double)r.ratio1 * (double)r.L) * s);
+    tA = (double)B.pd_tof * k * s;
+    tB = ((off * (double)B.pd_tof - (double)B.p1) * k - (double)r.ratio1 * (double)r.L) * s;
+}
+
+template <bool TOF>
+__device__ __forceinline__ void tof_coeffs(const Branch &B, const RaySetup &r, const LaunchGeom &G, float &tA,
+                                           float &tB)
+{
+    double a, b;
+    tof_coeffs_d<TOF>(B, r, G, a, b);
+    tA = (float)a;
+    tB = (float)b;
 }
 
 template <bool TOF>
 __global__ __launch_bounds__(BIN_THREADS) void k_bin_count(const float *__restrict__ lors, int64_t n, int stride,
                                                            const float *__restrict__ tofv, int tof_stride,
-                                                           LaunchGeom G, TileCfg T, int *__restrict__ ev_cnt,
+                                                           LaunchGeom G, TileCfg T, float twin,
+                                                           int *__restrict__ ev_cnt,
                                                            uint8_t *__restrict__ ev_flag,
                                                            unsigned *__restrict__ key_count)
 {
@@ -196,7 +231,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_count(const float *__restri
         RaySetup r;
         ray_setup<TOF>(lors + e * stride, TOF ? tofv[e * tof_stride] : 0.0f, G, r);
         int cnt = 0;
-        for_each_event_segment<TOF>(r, G, T, [&](int tile, int cls, int, int, const Branch &, const SegCtx &) {
+        for_each_event_segment<TOF>(r, G, T, twin, [&](int tile, int cls, int, int, const Branch &, const SegCtx &) {
             atomicAdd(&hist[tile * NCLASS + cls], 1u);
             ++cnt;
         });
@@ -211,7 +246,8 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_count(const float *__restri
 template <bool TOF>
 __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restrict__ lors, int64_t n, int stride,
                                                           const float *__restrict__ tofv, int tof_stride,
-                                                          LaunchGeom G, TileCfg T, const int *__restrict__ ev_off,
+                                                          LaunchGeom G, TileCfg T, float twin,
+                                                          const int *__restrict__ ev_off,
                                                           unsigned *__restrict__ key_cursor,
                                                           SegRec *__restrict__ recs, int *__restrict__ rec_ev,
                                                           unsigned long long *__restrict__ rec_sk)
@@ -224,7 +260,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
     const bool act = e < n;
     if (act) {
         ray_setup<TOF>(lors + e * stride, TOF ? tofv[e * tof_stride] : 0.0f, G, r);
-        for_each_event_segment<TOF>(r, G, T, [&](int tile, int cls, int, int, const Branch &, const SegCtx &) {
+        for_each_event_segment<TOF>(r, G, T, twin, [&](int tile, int cls, int, int, const Branch &, const SegCtx &) {
             atomicAdd(&hist[tile * NCLASS + cls], 1u);
         });
     }
@@ -234,7 +270,8 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
     __syncthreads();
     if (act) {
         int j = ev_off[e];
-        for_each_event_segment<TOF>(r, G, T, [&](int tile, int cls, int a, int b, const Branch &B, const SegCtx &c) {
+        for_each_event_segment<TOF>(r, G, T, twin, [&](int tile, int cls, int a, int b, const Branch &B,
+                                                       const SegCtx &c) {
             const unsigned pos = atomicAdd(&hist[tile * NCLASS + cls], 1u);
             SegRec R;
             R.p1 = B.p1;
@@ -273,10 +310,10 @@ __device__ __forceinline__ int cvt_flr(float v)
 enum : int { M_GATHER = 0, M_SCATTER = 1, M_DIGEST = 2, M_SCATTER_FX = 3 };
 
 // c * 2^shift as a 64-bit two's complement integer, exactly split into 32-bit
-// halves in float arithmetic (|c * 2^shift| < 2^41 by the choice of shift):
+// halves in float arithmetic (|c * 2^shift| < 2^49 by the choice of shift):
 // hi = floor(cs / 2^32) is an integer-valued float, lo = cs - hi * 2^32 lies in
 // [0, 2^32) and is exact (it keeps cs's 24-bit mantissa); the fraction below one
-// unit (2^-shift, at most 2^-40 of the largest term) is truncated.
+// unit (2^-shift, 2^-49 of the item's largest possible term) is truncated.
 __device__ __forceinline__ unsigned long long to_fixed(float c, int shift)
 {
     const float cs = ldexpf(c, shift);
@@ -917,16 +954,19 @@ extern "C" {
 int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const float *tof_values, int64_t tof_stride,
                     srf_grid_t grid, const srf_tof_t *tof, srf_plan **out, void *stream)
 {
-    return srf_plan_create_ex(lors, n, lor_stride, tof_values, tof_stride, grid, tof, ITEM, out, stream);
+    return srf_plan_create_ex(lors, n, lor_stride, tof_values, tof_stride, grid, tof, nullptr, out, stream);
 }
 
 int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const float *tof_values,
-                       int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof, int32_t item_records,
+                       int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof, const srf_plan_opts_t *opts,
                        srf_plan **out, void *stream)
 {
     if (!out) return SRF_ERR_INVALID_ARG;
     *out = nullptr;
+    const int32_t item_records = (opts && opts->item_records) ? opts->item_records : ITEM;
+    const float twin = opts ? opts->tof_window : 0.0f;
     if (item_records < 256 || item_records > ITEM) return SRF_ERR_INVALID_ARG;
+    if (!(twin >= 0.0f) || twin > 1000.0f) return SRF_ERR_INVALID_ARG;
     const unsigned item_max = (unsigned)item_records;
     if (n < 0 || (n > 0 && (!lors || lor_stride < 6))) return SRF_ERR_INVALID_ARG;
     if (tof && n > 0 && (!tof_values || tof_stride < 1)) return SRF_ERR_INVALID_ARG;
@@ -976,10 +1016,10 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
     if (n > 0) {
         if (P->tof)
             hipLaunchKernelGGL(k_bin_count<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, ev_cnt, P->ev_flag, key_count);
+                               tof_values, (int)tof_stride, G, T, twin, ev_cnt, P->ev_flag, key_count);
         else
             hipLaunchKernelGGL(k_bin_count<false>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, ev_cnt, P->ev_flag, key_count);
+                               tof_values, (int)tof_stride, G, T, twin, ev_cnt, P->ev_flag, key_count);
     }
     SRF_HIP(hipGetLastError());
     // event offsets: exclusive scan of the per-event counts (ev_off[n] = total)
@@ -1051,10 +1091,12 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
             return fail(SRF_ERR_LAUNCH);
         if (P->tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs, P->rec_ev, sk);
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_off, key_cursor, P->recs, P->rec_ev,
+                               sk);
         else
             hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, P->ev_off, key_cursor, P->recs, P->rec_ev, sk);
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_off, key_cursor, P->recs, P->rec_ev,
+                               sk);
         // Deterministic record order: the emit places each record inside its key's
         // range through atomic cursors, so the order within a key (and with it the
         // split into work items and each item's fixed-point scale) would vary from

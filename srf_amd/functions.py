@@ -137,7 +137,7 @@ class EmEngine:
     otherwise the one-event-per-lane kernels run on the raw LOR array."""
 
     def __init__(self, lors_data, values, emap_data, shape, geom, tof=None, ratio_kind=0, eps=0.0,
-                 zero_guard=0.0, tiled=True, plan=True, item_records=8192):
+                 zero_guard=0.0, tiled=True, plan=True, item_records=8192, tof_window=0.0):
         self.lors = to_gpu(lors_data)
         self.y = to_gpu(values).reshape(-1)
         self.shape = [int(v) for v in shape]
@@ -151,11 +151,12 @@ class EmEngine:
         self.zero_guard = float(zero_guard)
         self.tiled = bool(tiled)
         self.item_records = int(item_records)
+        self.tof_window = float(tof_window)
         self.plan = self.make_plan(self.lors) if (self.tiled and plan) else None
 
     def make_plan(self, lors):
         return ops.Plan(lors, self.shape, self.geom, self.use_tof, self.tof_bin, self.tof_sigma2,
-                        item_records=self.item_records)
+                        item_records=self.item_records, tof_window=self.tof_window)
 
     def forward(self, lors=None, plan=None):
         if plan is None and lors is None:
@@ -267,17 +268,20 @@ class OsemEngine(EmEngine):
     with ``tiled`` each subset is binned into its own plan at construction."""
 
     def __init__(self, lors_data, values, emap_data, shape, geom, n_sub, perm, tof=None, ratio_kind=0,
-                 eps=1e-8, zero_guard=0.0, tiled=True, item_records=8192):
+                 eps=1e-8, zero_guard=0.0, tiled=True, item_records=8192, tof_window=0.0):
         lors = to_gpu(lors_data)
         y = to_gpu(values).reshape(-1)
         n = lors.shape[0]
         self.n_sub = int(n_sub)
         self.n_per_sub = n // self.n_sub          # osem_full.py:209 (remainder dropped)
-        pidx = torch.as_tensor(np.asarray(perm[: self.n_per_sub * self.n_sub]), device=lors.device,
-                               dtype=torch.long)
+        if isinstance(perm, torch.Tensor):
+            pidx = perm[: self.n_per_sub * self.n_sub].to(lors.device, torch.long)
+        else:
+            pidx = torch.as_tensor(np.asarray(perm[: self.n_per_sub * self.n_sub]), device=lors.device,
+                                   dtype=torch.long)
         super().__init__(lors.index_select(0, pidx).contiguous(), y.index_select(0, pidx).contiguous(),
                          emap_data, shape, geom, tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard,
-                         tiled=tiled, plan=False, item_records=item_records)
+                         tiled=tiled, plan=False, item_records=item_records, tof_window=tof_window)
         del lors, y
         self.plans = [self.make_plan(self.subset(k)[0]) for k in range(self.n_sub)] if self.tiled else None
 

@@ -16,7 +16,7 @@ from typing import List, Optional, Tuple
 import torch
 
 from . import _lib
-from ._lib import SrfGrid, SrfTof, check, lib, stream_handle
+from ._lib import SrfGrid, SrfPlanOpts, SrfTof, check, lib, stream_handle
 
 Tensor = torch.Tensor
 
@@ -258,9 +258,10 @@ class Plan:
     round-off.  Device memory is owned by the plan and freed by close()/GC."""
 
     def __init__(self, lors: Tensor, shape, geom, use_tof: bool, tof_bin: float = 0.0, tof_sigma2: float = 0.0,
-                 tof_values: Optional[Tensor] = None, item_records: int = 8192):
+                 tof_values: Optional[Tensor] = None, item_records: int = 8192, tof_window: float = 0.0):
         n, stride, tp, ts = _lor_args(lors, tof_values, use_tof)
         self.item_records = int(item_records)
+        self.tof_window = float(tof_window)
         self.shape = [int(v) for v in shape]
         self.geom = [float(v) for v in geom]
         self.use_tof, self.tof_bin, self.tof_sigma2 = bool(use_tof), float(tof_bin), float(tof_sigma2)
@@ -268,8 +269,9 @@ class Plan:
         self.device = lors.device
         self._h = ctypes.c_void_p()
         with torch.cuda.device(lors.device):
+            opts = SrfPlanOpts(self.item_records, self.tof_window)
             rc = lib().srf_plan_create_ex(lors.data_ptr(), n, stride, tp, ts, _grid(self.shape, self.geom),
-                                          _tof(use_tof, tof_bin, tof_sigma2), self.item_records,
+                                          _tof(use_tof, tof_bin, tof_sigma2), ctypes.byref(opts),
                                           ctypes.byref(self._h), stream_handle(lors.device))
         check(rc, "srf_plan_create_ex")
         self._lib = lib()

@@ -4,7 +4,7 @@ the loop: the plain per-scan method and the batched method that replaces it.
 Per scan (both methods): values y_e = 1 / a_e (attenuation factors of the
 mu-map along each event, AttenuationCorrect semantics with the physical line
 integral); OSEM with n_sub subsets (OsemEngine semantics: seeded permutation,
-q = y / (A x + 1e-8), unguarded emap); before every iteration after the first,
+q = y / (A x + 1e-8), emap zeros guarded by 1e8); before every iteration after the first,
 a scatter sinogram is estimated from the current image and the mu-map and the
 values become (1 - s_e / sum(s) * N * fraction) / a_e (ScatterCorrect,
 srfnef/corrections/scattering/scatter.py:51-53; srf_scatter_correct).
@@ -29,15 +29,21 @@ import numpy as np
 import torch
 
 from . import ops, phantom
-from .batch import BatchConfig, BatchOsem, Scan
+from .batch import BatchConfig, BatchOsem, Scan, scan_permutation
 from .data import TofConfig, Umap
-from .functions import OsemEngine, image_to_kernel, subset_permutation, uniform_cylinder_lors
+from .functions import OsemEngine, uniform_cylinder_lors
 from .sss import SingleScatterSimulation, SssGeometry, scatter_correct
 from .surrogate import ScatterSurrogate, SinogramFeatures, relative_l2
 
 RADIUS, AXIAL = 424.0, 300.0
 TOF_BIN = 3.66
 FWHM = 60.0
+# Image grids may extend past the scanner's axial length, where the sensitivity
+# image is 0: those voxels take the guard of mlem.py:33-35 instead of 0/0.
+ZERO_GUARD = 1e8
+# TOF window of the plans (srf_plan_opts_t.tof_window): terms below 2^-26 of
+# the event's TOF peak are not binned (< 2e-9 of its weight).
+TOF_WINDOW = 26.0
 
 
 def _log(*a):
@@ -80,9 +86,19 @@ def make_scans(n_scans=32, seed=5, ev_range=(1e5, 2e7), sizes=(96, 128, 192, 256
     return scans
 
 
+def body_estimate(x_xyz, mu_xyz):
+    """The emission estimate restricted to the body (mu > 0): activity outside
+    tissue is unphysical, and early OSEM iterates spread some there."""
+    return x_xyz * (mu_xyz > 0).to(x_xyz.dtype)
+
+
 class ScatterEstimator:
     """Scatter sinograms for scans of any grid on one virtual-detector
-    geometry: full SSS, or the surrogate with an SSS check on `check_rings`."""
+    geometry: full SSS, or the surrogate with an SSS check on `check_rings`,
+    both from the emission estimate restricted to the body (body_estimate).
+    SSS is cell-volume normalised (the same volume integral on every grid;
+    identical to the plain sum on the surrogate's 4 mm training grid, where
+    the scatter point cell is exactly 8 mm)."""
 
     def __init__(self, geometry=None, net=None, kappa=None, check_rings=(16,), tol=0.02, fraction=0.3):
         self.g = geometry or SssGeometry()
@@ -97,14 +113,16 @@ class ScatterEstimator:
         return (tuple(sc.shape), tuple(sc.center), tuple(sc.size))
 
     def sss(self, sc, x_xyz):
+        x_xyz = body_estimate(x_xyz, sc.meta["mu"])
         k = self._key(sc)
         if k not in self._sss:
-            self._sss[k] = SingleScatterSimulation(self.g, sc.shape, sc.center, sc.size)
+            self._sss[k] = SingleScatterSimulation(self.g, sc.shape, sc.center, sc.size, cell_normalized=True)
         return self._sss[k](x_xyz, sc.meta["mu"])
 
     def surrogate(self, sc, x_xyz, check, scan_id):
         if self.fallback.get(scan_id):
             return self.sss(sc, x_xyz), True
+        x_xyz = body_estimate(x_xyz, sc.meta["mu"])
         k = self._key(sc)
         if k not in self._feat:
             self._feat[k] = SinogramFeatures(self.g, sc.shape, sc.center, sc.size, self.net.dev)
@@ -114,7 +132,8 @@ class ScatterEstimator:
         pred = (out * (m / self.kappa)).reshape(feat.dims)
         if check and self.check_rings:
             if k not in self._chk:
-                self._chk[k] = SingleScatterSimulation(self.g, sc.shape, sc.center, sc.size, rings=self.check_rings)
+                self._chk[k] = SingleScatterSimulation(self.g, sc.shape, sc.center, sc.size, rings=self.check_rings,
+                                                       cell_normalized=True)
             ref = self._chk[k](x_xyz, sc.meta["mu"])
             err = relative_l2(pred[list(self.check_rings)], ref)
             self.checks.append((scan_id, err))
@@ -134,22 +153,28 @@ def reconstruct_plain(scans, n_sub=8, n_iter=4, estimator=None, scatter="sss", i
     s2 = phantom.tof_sigma2_from_fwhm(FWHM)
     tof = TofConfig(TOF_BIN, s2)
     imgs = []
+    t_sc = 0.0
     torch.cuda.synchronize()
     t0 = time.time()
     for i, sc in enumerate(scans):
         n = sc.lors.shape[0]
-        perm = subset_permutation(n, sc.seed)
+        perm = scan_permutation(n, sc.seed, sc.lors.device)
         eng = OsemEngine(sc.lors, sc.meta["inv_att"], sc.emap, sc.shape, sc.geom, n_sub, perm, tof=tof,
-                         ratio_kind=0, eps=1e-8, zero_guard=0.0, item_records=item_records)
-        pidx = torch.as_tensor(perm[: eng.n_per_sub * n_sub], device=sc.lors.device, dtype=torch.long)
+                         ratio_kind=0, eps=1e-8, zero_guard=ZERO_GUARD, item_records=item_records,
+                         tof_window=TOF_WINDOW)
+        pidx = perm[: eng.n_per_sub * n_sub]
         for it in range(n_iter):
             if it >= 1 and scatter is not None:
+                torch.cuda.synchronize()
+                ts = time.time()
                 x = eng.image()
                 if scatter == "sss":
                     sino = estimator.sss(sc, x)
                 else:
                     sino, _ = estimator.surrogate(sc, x, it == 1, i)
                 eng.y.copy_(estimator.values(sc, sino).index_select(0, pidx))
+                torch.cuda.synchronize()
+                t_sc += time.time() - ts
             eng.run(n_sub)
         imgs.append(eng.image().clone())
         for p in eng.plans:
@@ -158,7 +183,7 @@ def reconstruct_plain(scans, n_sub=8, n_iter=4, estimator=None, scatter="sss", i
     torch.cuda.synchronize()
     t = time.time() - t0
     if log:
-        log(f"plain ({scatter}): {t:.2f} s")
+        log(f"plain ({scatter}): {t:.2f} s, of which scatter estimates {t_sc:.2f} s")
     return imgs, t
 
 
@@ -170,12 +195,15 @@ def reconstruct_batched(scans, n_sub=8, n_iter=4, estimator=None, config=BatchCo
     t0 = time.time()
     for sc in scans:
         sc.values = sc.meta["inv_att"]
-    b = BatchOsem(scans, n_sub, TOF_BIN, s2, True, config=config)
+    b = BatchOsem(scans, n_sub, TOF_BIN, s2, True, config=config, zero_guard=ZERO_GUARD, tof_window=TOF_WINDOW)
     t_setup = time.time() - t0
     if graph:
         b.capture()
+    t_sc = 0.0
     for it in range(n_iter):
         if it >= 1 and scatter is not None:
+            torch.cuda.synchronize()
+            ts = time.time()
             for i, sc in enumerate(scans):
                 x = b.image(i)
                 if scatter == "sss":
@@ -183,6 +211,8 @@ def reconstruct_batched(scans, n_sub=8, n_iter=4, estimator=None, config=BatchCo
                 else:
                     sino, _ = estimator.surrogate(sc, x, it == 1, i)
                 b.set_values(i, estimator.values(sc, sino))
+            torch.cuda.synchronize()
+            t_sc += time.time() - ts
         if graph:
             b.graph.replay()
         else:
@@ -192,9 +222,10 @@ def reconstruct_batched(scans, n_sub=8, n_iter=4, estimator=None, config=BatchCo
     t = time.time() - t0
     info = b.info(0)
     info["setup_s"] = round(t_setup, 3)
+    info["scatter_s"] = round(t_sc, 3)
     b.close()
     if log:
-        log(f"batched ({scatter}, graph={graph}): {t:.2f} s (setup {t_setup:.2f} s)")
+        log(f"batched ({scatter}, graph={graph}): {t:.2f} s (plans {t_setup:.2f} s, scatter estimates {t_sc:.2f} s)")
     return imgs, t, info
 
 

@@ -121,25 +121,43 @@ class SssGeometry:
                         self.n_rings - 1)
         return (r * self.n_angles + a) * self.n_radial + rho
 
-    def scatter_points(self, mu_image, center, size, device):
-        """(ns, 4): grid points of preprocess.py:6-18 (step = floor(spacing /
-        voxel) voxels) with mu at the point's voxel; points outside the body
-        carry mu = 0 and drop out of the sum."""
-        shape = mu_image.shape
+    def point_grid(self, shape, center, size, device, physical=False):
+        """Scatter point grid: positions (ns, 3) float32, the flat (x, y, z)-
+        order index of each point's voxel (ns,) int64, and the cell volume in
+        mm^3.
+
+        physical=False: preprocess.py:6-18 -- step = floor(spacing / voxel)
+        voxels, so the points (and the sum) depend on the image grid.
+        physical=True: a point every `spacing` mm from the image's lower
+        corner, mu from the voxel holding it; the same points on every grid
+        (identical to the first where spacing is a whole number of voxels)."""
         vox = [size[i] / shape[i] for i in range(3)]
-        step = [max(1, int(math.floor(self.spacing / vox[i]))) for i in range(3)]
-        cnt = [int(shape[i] // step[i]) for i in range(3)]
-        g = [torch.arange(cnt[i], dtype=torch.float64) for i in range(3)]
-        X, Y, Z = torch.meshgrid(*g, indexing="ij")
-        pos = torch.stack([step[0] * (X + 0.5) * vox[0] - size[0] / 2 + center[0],
-                           step[1] * (Y + 0.5) * vox[1] - size[1] / 2 + center[1],
-                           step[2] * (Z + 0.5) * vox[2] - size[2] / 2 + center[2]], -1).reshape(-1, 3)
-        ix = (step[0] * X + step[0] // 2).long().reshape(-1)
-        iy = (step[1] * Y + step[1] // 2).long().reshape(-1)
-        iz = (step[2] * Z + step[2] // 2).long().reshape(-1)
-        mu = torch.as_tensor(mu_image, device="cpu").to(torch.float64)[ix, iy, iz]
-        out = torch.cat([pos, mu[:, None]], 1).to(torch.float32)
-        return out.to(device)
+        if physical:
+            cnt = [max(1, int(math.floor(size[i] / self.spacing + 1e-9))) for i in range(3)]
+            g = [torch.arange(cnt[i], dtype=torch.float64) for i in range(3)]
+            X, Y, Z = torch.meshgrid(*g, indexing="ij")
+            off = [(t + 0.5) * self.spacing for t in (X, Y, Z)]
+            idx = [torch.clamp(torch.floor(off[i] / vox[i] + 1e-9), 0, shape[i] - 1).long().reshape(-1)
+                   for i in range(3)]
+            cell = self.spacing ** 3
+        else:
+            step = [max(1, int(math.floor(self.spacing / vox[i]))) for i in range(3)]
+            cnt = [int(shape[i] // step[i]) for i in range(3)]
+            g = [torch.arange(cnt[i], dtype=torch.float64) for i in range(3)]
+            X, Y, Z = torch.meshgrid(*g, indexing="ij")
+            off = [step[i] * (t + 0.5) * vox[i] for i, t in enumerate((X, Y, Z))]
+            idx = [(step[i] * t + step[i] // 2).long().reshape(-1) for i, t in enumerate((X, Y, Z))]
+            cell = step[0] * vox[0] * step[1] * vox[1] * step[2] * vox[2]
+        pos = torch.stack([off[i] - size[i] / 2 + center[i] for i in range(3)], -1).reshape(-1, 3)
+        flat = (idx[0] * shape[1] + idx[1]) * shape[2] + idx[2]
+        return pos.to(torch.float32).to(device), flat.to(device), cell
+
+    def scatter_points(self, mu_image, center, size, device):
+        """(ns, 4): grid points with mu at the point's voxel; points outside
+        the body carry mu = 0 and drop out of the sum."""
+        mu = torch.as_tensor(mu_image).to(device, torch.float32)
+        pos, flat, _ = self.point_grid(mu.shape, center, size, device)
+        return torch.cat([pos, mu.reshape(-1)[flat][:, None]], 1).contiguous()
 
 
 class SingleScatterSimulation:
@@ -152,7 +170,8 @@ class SingleScatterSimulation:
     rings: evaluate only the bins of these rings (their detectors only);
     the output is then (len(rings), n_angles, n_radial)."""
 
-    def __init__(self, geometry, shape, center, size, physical_kn=False, tiled=False, compact=True, rings=None):
+    def __init__(self, geometry, shape, center, size, physical_kn=False, tiled=False, compact=True, rings=None,
+                 cell_normalized=False, physical_grid=False):
         self.g = geometry
         self.shape = [int(v) for v in shape]
         self.center = [float(v) for v in center]
@@ -176,8 +195,8 @@ class SingleScatterSimulation:
         self.n_out_rings = len(rings) if rings is not None else geometry.n_rings
         self.dx_ = self.size[0] / self.shape[0]
         # the full scatter grid (mu decides which points count)
-        grid = geometry.scatter_points(np.zeros(self.shape, np.float32), self.center, self.size, self.dev)
-        self.pts_xyz = grid[:, :3].contiguous()
+        self.pts_xyz, self.pts_flat, self.cell = geometry.point_grid(self.shape, self.center, self.size, self.dev,
+                                                                     physical=physical_grid)
         self.ns = self.pts_xyz.shape[0]
         self.plan = None
         if not self.compact:
@@ -189,7 +208,11 @@ class SingleScatterSimulation:
         self.tables = torch.from_numpy(energy_tables(geometry.low_energy, geometry.high_energy,
                                                      geometry.resolution, physical_kn)).to(self.dev)
         self.scale = sss_scale(geometry.low_energy, geometry.high_energy, geometry.resolution)
-        self._rows = None
+        if cell_normalized:
+            # weight each point by its cell volume in units of a spacing^3 cell: the
+            # sum then approximates the same volume integral on every image grid
+            # (the factor is 1 where the point step times the voxel equals spacing)
+            self.scale *= self.cell / geometry.spacing ** 3
 
     def _lors(self, pts_xyz):
         ns = pts_xyz.shape[0]
@@ -218,16 +241,17 @@ class SingleScatterSimulation:
         return self._integrals(img_xyz, lors, ti)
 
     def points(self, mu_xyz):
-        m = self.g.scatter_points(mu_xyz.detach().cpu().numpy(), self.center, self.size, self.dev)
+        mu = mu_xyz.detach().to(self.dev, torch.float32).reshape(-1)[self.pts_flat]
+        m = torch.cat([self.pts_xyz, mu[:, None]], 1)
         if self.compact:
-            m = m[m[:, 3] != 0]
+            m = m[mu != 0]
         return m.contiguous()
 
     def _rows_for(self, ns):
+        # a fresh workspace per call (the caching allocator recycles it): an
+        # estimator that keeps one simulation per image grid holds no memory
         need = int(lib().srf_sss_rows_bytes(self.nd, max(ns, 1))) // 4 + 4
-        if self._rows is None or self._rows.numel() < need:
-            self._rows = torch.empty(need, dtype=torch.float32, device=self.dev)
-        return self._rows
+        return torch.empty(need, dtype=torch.float32, device=self.dev)
 
     def __call__(self, emission_xyz, mu_xyz):
         """(n_rings or len(rings), n_angles, n_radial) float32 scatter sinogram."""

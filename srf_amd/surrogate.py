@@ -408,5 +408,55 @@ def train(net, X, T, dims, epochs, batch=4, lr=1e-3, seed=0, log=None, lr_min_fr
     return hist
 
 
+def flops_per_pass(net, dims):
+    """Dense MFMA FLOPs of one forward pass (real channels; 2 per multiply-add)."""
+    D, H, W = dims
+    total = 0
+    for l in net.layers:
+        lvl = int(l.name[3]) if l.name[:3] in ("enc", "dec") else 0
+        V = (D >> lvl) * (H >> lvl) * (W >> lvl)
+        total += 2 * V * l.taps * l.cin * l.cout
+    return total
+
+
+def _main():
+    """Time forward and training passes of the surrogate network on random
+    bf16 features at the sinogram grid (32 x 64 x 64)."""
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--dims", type=int, nargs=3, default=(32, 64, 64))
+    a = ap.parse_args()
+    dev = torch.device("cuda", 0)
+    dims = tuple(a.dims)
+    V = dims[0] * dims[1] * dims[2]
+    net = UNet(device=dev)
+    x = to_bf16(torch.randn((V, 8), device=dev))
+    t_ = torch.rand(V, device=dev)
+    for _ in range(2):
+        net.forward(x, dims)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        net.forward(x, dims)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / a.steps
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        net.zero_grad()
+        out = net.forward(x, dims)
+        net.backward(net.loss_and_grad(out, t_, dims), dims)
+        net.adam(1e-4)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / a.steps
+    f = flops_per_pass(net, dims)
+    print(f"dims {dims}: forward {1e3 * t_fwd:.3f} ms ({f / t_fwd / 1e12:.1f} TFLOP/s dense), "
+          f"training step {1e3 * t_step:.3f} ms ({3 * f / t_step / 1e12:.1f} TFLOP/s)", flush=True)
+
+
+if __name__ == "__main__":
+    _main()
+
+
 __all__ = ["UNet", "SinogramFeatures", "ScatterSurrogate", "train", "relative_l2", "to_bf16",
-           "bf16_bits_to_float", "unet_layers"]
+           "bf16_bits_to_float", "unet_layers", "flops_per_pass"]

@@ -39,17 +39,120 @@ def workload(device):
     return g, center, size, sss, feat
 
 
-def make_pairs(seeds, sss, feat, device, log=_log):
-    """Features X (n, V, 8) bf16 bits, SSS sinograms S (n, V), emission means M (n,)."""
+def varied_case(seed, device, axial_fov=300.0):
+    """An (emission, mu) pair like the ones met inside a batched OSEM loop:
+    a random grid (in-plane {96,128,192,256}, 64-128 slices, isotropic 2-4 mm
+    voxels), the OSEM workload's random ellipsoid phantom or a scatter
+    phantom, and an emission that looks like an early OSEM estimate (voxel
+    noise of random strength, usually no activity beyond the scanner's axial
+    field of view).  Returns (emission, mu, shape, center, size)."""
+    rng = np.random.default_rng(seed)
+    nxy = int(rng.choice((96, 128, 192, 256)))
+    nz = int(rng.integers(64, 129))
+    vox = float(rng.uniform(2.0, 4.0))
+    shape = (nxy, nxy, nz)
+    if rng.uniform() < 0.5:
+        act, mu, center, size = phantom.random_ellipsoid_phantom(shape, vox, int(seed), device=device)
+    else:
+        act, mu, center, size = phantom.scatter_phantom(shape, vox, int(seed), device=device)
+    gen = torch.Generator(device=device).manual_seed(int(seed))
+    sigma = float(rng.uniform(0.0, 0.8))
+    act = torch.clamp(act * (1.0 + sigma * torch.randn(act.shape, generator=gen, device=device)), min=0.0)
+    if rng.uniform() < 0.7:
+        z = (torch.arange(nz, device=device, dtype=torch.float32) + 0.5) * vox - size[2] / 2
+        act = act * (z.abs() <= axial_fov / 2).to(act.dtype)[None, None, :]
+    return act, mu, shape, center, size
+
+
+def osem_cases(seed, device, n_iter=3):
+    """(emission, mu, shape, center, size) pairs as met inside a batched OSEM
+    loop: a random grid as in varied_case, the OSEM workload's random
+    ellipsoid phantom or a scatter phantom, TOF events (log-uniform 1e5 to
+    4e6, FWHM 60 mm) with attenuation-corrected values, and the OSEM
+    estimates after iterations 1..n_iter (8 subsets) restricted to the body
+    (mu > 0), as pipeline.ScatterEstimator sees them."""
+    import math
+    from .batch import BatchOsem, Scan
+    from .corrections import attenuation_factors
+    from .data import Umap
+    from .functions import uniform_cylinder_lors
+    from . import ops
+    from .pipeline import AXIAL, FWHM, RADIUS, TOF_BIN, ZERO_GUARD, body_estimate
+    rng = np.random.default_rng(seed)
+    nxy = int(rng.choice((96, 128, 192, 256)))
+    nz = int(rng.integers(64, 129))
+    vox = float(rng.uniform(2.0, 4.0))
+    shape = (nxy, nxy, nz)
+    if rng.uniform() < 0.5:
+        act, mu, center, size = phantom.random_ellipsoid_phantom(shape, vox, int(seed), device=device)
+    else:
+        act, mu, center, size = phantom.scatter_phantom(shape, vox, int(seed), device=device)
+    n = int(round(math.exp(rng.uniform(math.log(1e5), math.log(4e6)))))
+    lors = phantom.sample_events(act, center, size, n, RADIUS, AXIAL, int(seed) + 7, tof_fwhm_mm=FWHM, device=device)
+    inv_att = (1.0 / attenuation_factors(Umap(mu, center, size), lors, physical=True)).contiguous()
+    n_e = 500_000
+    el = uniform_cylinder_lors(n_e, RADIUS, AXIAL, int(seed) + 11, device)
+    emap = ops.from_kernel_layout(ops.backproject(el, None, list(shape), list(center) + list(size), False, 0.0, 0.0,
+                                                  0)) / n_e
+    b = BatchOsem([Scan(lors, shape, center, size, emap, values=inv_att, seed=int(seed))], 8, TOF_BIN,
+                  phantom.tof_sigma2_from_fwhm(FWHM), True, zero_guard=ZERO_GUARD)
+    out = []
+    for _ in range(n_iter):
+        b.iteration()
+        out.append((body_estimate(b.image(0), mu), mu, shape, center, size))
+    b.close()
+    return out
+
+
+def make_osem_pairs(n_pairs, seed0, sss_geometry, device, log=_log, n_iter=3):
+    """n_pairs in-loop pairs (n_iter per simulated scan): features, cell-
+    normalised SSS sinograms and emission means, as make_pairs returns."""
+    from .surrogate import SinogramFeatures as SF
+    V = sss_geometry.n_rings * sss_geometry.n_angles * sss_geometry.n_radial
+    X = torch.empty((n_pairs, V, 8), dtype=torch.int16, device=device)
+    S = torch.empty((n_pairs, V), dtype=torch.float32, device=device)
+    M = torch.empty(n_pairs, dtype=torch.float32, device=device)
+    i, k, t0 = 0, 0, time.time()
+    while i < n_pairs:
+        for em, mu, shape, center, size in osem_cases(seed0 + k, device, n_iter):
+            if i >= n_pairs:
+                break
+            s_i = SingleScatterSimulation(sss_geometry, shape, center, size, cell_normalized=True)
+            f_i = SF(sss_geometry, shape, center, size, device)
+            S[i] = s_i(em, mu).reshape(-1)
+            x, m = f_i(em, mu)
+            X[i] = x
+            M[i] = m[0]
+            i += 1
+            if i % 200 == 0 or i == n_pairs:
+                torch.cuda.synchronize()
+                log(f"in-loop pairs {i}/{n_pairs} {time.time() - t0:.0f}s")
+        k += 1
+    return X, S, M
+
+
+def make_pairs(seeds, sss, feat, device, log=_log, varied=None):
+    """Features X (n, V, 8) bf16 bits, SSS sinograms S (n, V), emission means
+    M (n,).  varied: None (all on the surrogate grid) or a per-seed flag list
+    selecting varied_case pairs, whose SSS is cell-volume normalised (equal
+    to the plain sum on the surrogate grid)."""
     n, V = len(seeds), feat.n
     X = torch.empty((n, V, 8), dtype=torch.int16, device=device)
     S = torch.empty((n, V), dtype=torch.float32, device=device)
     M = torch.empty(n, dtype=torch.float32, device=device)
+    g = sss.g
     t0 = time.time()
     for i, seed in enumerate(seeds):
-        act, mu, _, _ = phantom.scatter_phantom(SHAPE, VOXEL, int(seed), device=device)
-        S[i] = sss(act, mu).reshape(-1)
-        x, m = feat(act, mu)
+        if varied is not None and varied[i]:
+            act, mu, shape, center, size = varied_case(int(seed), device)
+            s_i = SingleScatterSimulation(g, shape, center, size, cell_normalized=True)
+            f_i = SinogramFeatures(g, shape, center, size, device)
+            S[i] = s_i(act, mu).reshape(-1)
+            x, m = f_i(act, mu)
+        else:
+            act, mu, _, _ = phantom.scatter_phantom(SHAPE, VOXEL, int(seed), device=device)
+            S[i] = sss(act, mu).reshape(-1)
+            x, m = feat(act, mu)
         X[i] = x
         M[i] = m[0]
         if (i + 1) % 200 == 0 or i + 1 == n:
@@ -75,6 +178,12 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--time-limit", type=float, default=None, help="seconds of training before stopping")
+    ap.add_argument("--varied", type=float, default=0.0,
+                    help="fraction of training pairs drawn by varied_case (random grids, noisy emission)")
+    ap.add_argument("--test-varied", type=int, default=0, help="held-out varied_case pairs")
+    ap.add_argument("--in-loop", action="store_true",
+                    help="train on OSEM estimates of simulated scans (osem_cases) instead of phantoms")
+    ap.add_argument("--test-in-loop", type=int, default=0, help="held-out in-loop pairs")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "weights",
                                                   "scatter_unet_128x128x64.npz"))
     a = ap.parse_args(argv)
@@ -82,8 +191,20 @@ def main(argv=None):
     g, center, size, sss, feat = workload(dev)
     dims = feat.dims
     _log(f"generating {a.train} training and {a.test} held-out pairs")
-    Xtr, Str, Mtr = make_pairs(range(a.train), sss, feat, dev)
+    rng = np.random.default_rng(a.seed + 17)
+    flags = list(rng.uniform(size=a.train) < a.varied) if a.varied > 0 else None
+    if a.in_loop:
+        Xtr, Str, Mtr = make_osem_pairs(a.train, 0, g, dev)
+    else:
+        Xtr, Str, Mtr = make_pairs(range(a.train), sss, feat, dev, varied=flags)
     Xte, Ste, Mte = make_pairs(range(HELD_OUT_SEED0, HELD_OUT_SEED0 + a.test), sss, feat, dev)
+    Xtl = None
+    if a.test_in_loop:
+        Xtl, Stl, Mtl = make_osem_pairs(a.test_in_loop, HELD_OUT_SEED0, g, dev)
+    Xtv = None
+    if a.test_varied:
+        vs = range(HELD_OUT_SEED0 + 500_000, HELD_OUT_SEED0 + 500_000 + a.test_varied)
+        Xtv, Stv, Mtv = make_pairs(vs, sss, feat, dev, varied=[True] * a.test_varied)
     # kappa puts the normalised targets at unit mean
     kappa = float(1.0 / (Str.mean(1) / Mtr).mean())
     Ttr = (Str / Mtr[:, None] * kappa).contiguous()
@@ -97,6 +218,19 @@ def main(argv=None):
     _log(f"train rel L2 (first 200): mean {err_tr.mean():.4f}")
     _log(f"held-out rel L2: mean {err.mean():.4f} median {np.median(err):.4f} p95 {np.percentile(err, 95):.4f} "
          f"max {err.max():.4f} frac<=2% {(err <= 0.02).mean():.3f}")
+    extra = {}
+    if Xtv is not None:
+        ev = evaluate(net, Xtv, Stv, Mtv, kappa, dims)
+        _log(f"held-out varied rel L2: mean {ev.mean():.4f} median {np.median(ev):.4f} "
+             f"p95 {np.percentile(ev, 95):.4f} max {ev.max():.4f} frac<=2% {(ev <= 0.02).mean():.3f}")
+        extra = dict(varied_test=len(ev), varied_mean=float(ev.mean()), varied_p95=float(np.percentile(ev, 95)),
+                     varied_max=float(ev.max()), varied_frac_le_2pct=float((ev <= 0.02).mean()))
+    if Xtl is not None:
+        el = evaluate(net, Xtl, Stl, Mtl, kappa, dims)
+        _log(f"held-out in-loop rel L2: mean {el.mean():.4f} median {np.median(el):.4f} "
+             f"p95 {np.percentile(el, 95):.4f} max {el.max():.4f} frac<=2% {(el <= 0.02).mean():.3f}")
+        extra.update(in_loop_test=len(el), in_loop_mean=float(el.mean()), in_loop_p95=float(np.percentile(el, 95)),
+                     in_loop_max=float(el.max()), in_loop_frac_le_2pct=float((el <= 0.02).mean()))
     # timing: surrogate (features + network) vs SSS on one held-out phantom
     act, mu, _, _ = phantom.scatter_phantom(SHAPE, VOXEL, HELD_OUT_SEED0, device=dev)
     sur = ScatterSurrogate(net, kappa, g, SHAPE, center, size, check_rings=())
@@ -119,7 +253,8 @@ def main(argv=None):
     net.save(a.out, kappa=kappa, shape=list(SHAPE), voxel=VOXEL, train=a.train, test=a.test,
              epochs=len(hist), batch=a.batch, lr=a.lr, train_seconds=round(t_train, 1),
              final_loss=float(hist[-1]), heldout_mean=float(err.mean()), heldout_max=float(err.max()),
-             heldout_p95=float(np.percentile(err, 95)))
+             heldout_p95=float(np.percentile(err, 95)), varied_train_fraction=a.varied, in_loop=bool(a.in_loop),
+             **extra)
     _log(f"saved {a.out}")
 
 
