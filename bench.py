@@ -235,6 +235,8 @@ def main():
 
 
 WEIGHTS = os.path.join(ROOT, "srf_amd", "weights", "scatter_unet_128x128x64.npz")
+# the model trained on OSEM estimates (python -m srf_amd.train_surrogate --in-loop), used inside the loop
+WEIGHTS_IN_LOOP = os.path.join(ROOT, "srf_amd", "weights", "scatter_unet_in_loop.npz")
 
 
 def _timed(fn):
@@ -308,21 +310,26 @@ def batch_bench(args):
     scans = make_scans(n_scans=args.batch_scans, seed=args.batch_seed, device=dev)
     n_ev = sum(int(s.lors.shape[0]) for s in scans)
     log(f"batch workload: {len(scans)} scans, {n_ev} events, setup {time.time() - t0:.1f}s")
-    net, meta = UNet.load(WEIGHTS, device=dev)
+    wpath = WEIGHTS_IN_LOOP if os.path.exists(WEIGHTS_IN_LOOP) else WEIGHTS
+    net, meta = UNet.load(wpath, device=dev)
     kappa = meta["kappa"]
     cache = os.environ.get("SRF_AUTOTUNE_CACHE", os.path.join(tempfile.gettempdir(), "srf_amd_autotune.json"))
     tuner = Autotuner(cache, reps=3)
     s2 = phantom.tof_sigma2_from_fwhm(FWHM)
     (cfg, t_tune) = _timed(lambda: tuner.tune(scans, 8, TOF_BIN, s2, True, tof_window=TOF_WINDOW))
     log(f"autotuned {cfg} in {t_tune:.1f}s")
-    imgs_plain, t_plain = reconstruct_plain(scans, 8, 4, ScatterEstimator(net=net, kappa=kappa), scatter="sss",
-                                            log=log)
+    cs = meta.get("coarse_spacing") or None
+
+    def estimator():
+        return ScatterEstimator(net=net, kappa=kappa, coarse_spacing=cs)
+
+    imgs_plain, t_plain = reconstruct_plain(scans, 8, 4, estimator(), scatter="sss", log=log)
     torch.cuda.empty_cache()
-    est_b = ScatterEstimator(net=net, kappa=kappa)
+    est_b = estimator()
     imgs_b, t_b, info = reconstruct_batched(scans, 8, 4, est_b, config=cfg, graph=True, log=log)
     torch.cuda.empty_cache()
-    imgs_p, t_p = reconstruct_plain(scans, 8, 4, ScatterEstimator(net=net, kappa=kappa), scatter="surrogate",
-                                    item_records=cfg.item_records, log=log)
+    imgs_p, t_p = reconstruct_plain(scans, 8, 4, estimator(), scatter="surrogate", item_records=cfg.item_records,
+                                    log=log)
     eq = [float((a - b).norm() / b.norm()) for a, b in zip(imgs_b, imgs_p)]
     bitwise = all(bool(torch.equal(a, b)) for a, b in zip(imgs_b, imgs_p))
     vs_sss = [float((a - b).norm() / b.norm()) for a, b in zip(imgs_b, imgs_plain)]
@@ -339,7 +346,8 @@ def batch_bench(args):
             "image_rel_l2_surrogate_vs_sss_scatter_mean": float(np.mean(vs_sss)),
             "check_err_max": max(checks) if checks else None,
             "fallback_scans": sum(1 for v in est_b.fallback.values() if v),
-            "autotuned": cfg.__dict__, "autotune_s": t_tune, "batch_info": info}
+            "autotuned": cfg.__dict__, "autotune_s": t_tune, "batch_info": info,
+            "surrogate_weights": os.path.basename(wpath), "tof_window": TOF_WINDOW}
 
 
 def cpu_baseline(eng, shape, center, size, tof_bin, tof_sigma2, sample):

@@ -309,16 +309,18 @@ int srf_cnn_act_bwd(const float *da, int32_t das, const void *a, int32_t as, int
                     float slope, void *dp, void *stream);
 /* db[c] += sum_v dp[v][c] for c < C (dp bf16 rows of Cp). */
 int srf_cnn_bias_grad(const void *dp, int64_t V, int32_t C, int32_t Cp, float *db, void *stream);
-/* loss[0] += mean((p - t)^2); grad = 2 (p - t) / n. */
-int srf_cnn_mse(const float *p, const float *t, int64_t n, float *grad, float *loss, void *stream);
+/* loss[0] += w mean((p - t)^2); grad = 2 w (p - t) / n  (w = 1 / mean(t^2): a
+ * relative loss). */
+int srf_cnn_mse(const float *p, const float *t, int64_t n, float weight, float *grad, float *loss, void *stream);
 /* One Adam step (Kingma & Ba, 2015) on FP32 master weights w with gradient
  * g * gscale; step counts from 1; wbf (may be NULL) receives bf16(w). */
 int srf_cnn_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps,
                  int32_t step, float gscale, void *wbf, void *stream);
-/* Surrogate input: x[i][0..7] = bf16 of [e, exp(-mu), e exp(-mu), mu/4, 0, 0, 0, 0]
- * with e = em_int[i] * scale[0] (scale is a device pointer). */
-int srf_cnn_features(const float *em_int, const float *mu_int, int64_t n, const float *scale, void *x,
-                     void *stream);
+/* Surrogate input: x[i][0..7] = bf16 of
+ * [e, exp(-mu), e exp(-mu), mu/4, extra[i] * scale[1], 0, 0, 0] with
+ * e = em_int[i] * scale[0] (scale is a device pointer; extra may be NULL: 0). */
+int srf_cnn_features(const float *em_int, const float *mu_int, const float *extra, int64_t n, const float *scale,
+                     void *x, void *stream);
 /* f32 <-> bf16 (round to nearest even). */
 int srf_cnn_cast(const void *x, int64_t n, int32_t to_bf16, void *y, void *stream);
 

@@ -102,9 +102,9 @@ SIGNATURES = [
     ("srf_cnn_copy_channels", ctypes.c_int, [_P, _I64, _I32, _I32, _I32, _P, _I32, _I32, _I32, _I32, _P]),
     ("srf_cnn_act_bwd", ctypes.c_int, [_P, _I32, _P, _I32, _I64, _I32, _I32, _F, _P, _P]),
     ("srf_cnn_bias_grad", ctypes.c_int, [_P, _I64, _I32, _I32, _P, _P]),
-    ("srf_cnn_mse", ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    ("srf_cnn_mse", ctypes.c_int, [_P, _P, _I64, _F, _P, _P, _P]),
     ("srf_cnn_adam", ctypes.c_int, [_P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _F, _P, _P]),
-    ("srf_cnn_features", ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    ("srf_cnn_features", ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P]),
     ("srf_cnn_cast", ctypes.c_int, [_P, _I64, _I32, _P, _P]),
 ]
 

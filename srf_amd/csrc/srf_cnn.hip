@@ -281,15 +281,15 @@ __global__ __launch_bounds__(256) void k_bias_grad(const unsigned short *__restr
     if (threadIdx.x == 0) db[c] += red[0];
 }
 
-// loss = mean((p - t)^2) accumulated into loss[0]; grad = 2 (p - t) / n
+// loss = w mean((p - t)^2) accumulated into loss[0]; grad = 2 w (p - t) / n
 __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ p, const float *__restrict__ t, int64_t n,
-                                             float *__restrict__ grad, float *__restrict__ loss)
+                                             float w, float *__restrict__ grad, float *__restrict__ loss)
 {
     __shared__ float red[256];
     float s = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const float e = p[i] - t[i];
-        grad[i] = 2.0f * e / (float)n;
+        grad[i] = 2.0f * w * e / (float)n;
         s += e * e;
     }
     red[threadIdx.x] = s;
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ p, const 
         if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0] / (float)n);
+    if (threadIdx.x == 0) atomicAdd(loss, w * red[0] / (float)n);
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master weights; writes the bf16 copy.
@@ -331,10 +331,12 @@ __global__ void k_bf16_to_f32(const unsigned short *__restrict__ x, int64_t n, f
 }
 
 // Surrogate input features per sinogram bin from the emission and attenuation
-// line integrals along the bin's LOR (scale[0] = 1 / mean emission integral):
-//   [e, exp(-mu), e * exp(-mu), mu / 4, 0, 0, 0, 0] with e = em * scale[0]
-__global__ void k_features(const float *__restrict__ em, const float *__restrict__ mu, int64_t n,
-                           const float *__restrict__ scale, unsigned short *__restrict__ x)
+// line integrals along the bin's LOR (scale[0] = 1 / mean emission integral)
+// and an optional extra sinogram (scale[1]):
+//   [e, exp(-mu), e * exp(-mu), mu / 4, extra * scale[1], 0, 0, 0], e = em * scale[0]
+__global__ void k_features(const float *__restrict__ em, const float *__restrict__ mu,
+                           const float *__restrict__ extra, int64_t n, const float *__restrict__ scale,
+                           unsigned short *__restrict__ x)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -345,7 +347,8 @@ __global__ void k_features(const float *__restrict__ em, const float *__restrict
     lo.y = f2bf(a);
     lo.z = f2bf(e * a);
     lo.w = f2bf(0.25f * mu[i]);
-    hi.x = hi.y = hi.z = hi.w = 0;
+    hi.x = extra ? f2bf(extra[i] * scale[1]) : (unsigned short)0;
+    hi.y = hi.z = hi.w = 0;
     reinterpret_cast<ushort4 *>(x)[2 * i] = lo;
     reinterpret_cast<ushort4 *>(x)[2 * i + 1] = hi;
 }
@@ -472,11 +475,11 @@ int srf_cnn_bias_grad(const void *dp, int64_t V, int32_t C, int32_t Cp, float *d
     return launch_status();
 }
 
-int srf_cnn_mse(const float *p, const float *t, int64_t n, float *grad, float *loss, void *stream)
+int srf_cnn_mse(const float *p, const float *t, int64_t n, float weight, float *grad, float *loss, void *stream)
 {
-    if (!p || !t || !grad || !loss || n <= 0) return SRF_ERR_INVALID_ARG;
+    if (!p || !t || !grad || !loss || n <= 0 || !(weight >= 0.0f)) return SRF_ERR_INVALID_ARG;
     hipLaunchKernelGGL(k_mse, dim3((unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024)), dim3(256), 0,
-                       (hipStream_t)stream, p, t, n, grad, loss);
+                       (hipStream_t)stream, p, t, n, weight, grad, loss);
     return launch_status();
 }
 
@@ -490,14 +493,14 @@ int srf_cnn_adam(float *w, const float *g, float *m, float *v, int64_t n, float 
     return launch_status();
 }
 
-int srf_cnn_features(const float *em_int, const float *mu_int, int64_t n, const float *scale, void *x,
-                     void *stream)
+int srf_cnn_features(const float *em_int, const float *mu_int, const float *extra, int64_t n, const float *scale,
+                     void *x, void *stream)
 {
     if (!em_int || !mu_int || !scale || !x || n < 0) return SRF_ERR_INVALID_ARG;
     if (reinterpret_cast<uintptr_t>(x) & 7) return SRF_ERR_INVALID_ARG;
     if (n == 0) return SRF_OK;
-    hipLaunchKernelGGL(k_features, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, em_int, mu_int, n, scale,
-                       (unsigned short *)x);
+    hipLaunchKernelGGL(k_features, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, em_int, mu_int, extra, n,
+                       scale, (unsigned short *)x);
     return launch_status();
 }
 

@@ -291,7 +291,11 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
             R.pad[0] = R.pad[1] = 0u;
             recs[pos] = R;
             rec_ev[pos] = (int)e;
-            rec_sk[pos] = ((unsigned long long)(tile * NCLASS + cls) << 32) | (unsigned)R.slot;
+            // sort key: (tile, class), then slab count (longest first: a wavefront's
+            // lanes march similar lengths), then the segment slot (unique)
+            const unsigned len = (unsigned)min(b - a, 63);
+            rec_sk[pos] = ((unsigned long long)(tile * NCLASS + cls) << 38) |
+                          ((unsigned long long)(63u - len) << 32) | (unsigned)R.slot;
         });
     }
 }
@@ -1100,8 +1104,9 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
         // Deterministic record order: the emit places each record inside its key's
         // range through atomic cursors, so the order within a key (and with it the
         // split into work items and each item's fixed-point scale) would vary from
-        // run to run.  Sorting by (key, segment slot) fixes it: every plan of the
-        // same events is bit-identical.
+        // run to run.  Sorting by (key, slab count, segment slot) fixes it -- every
+        // plan of the same events is bit-identical -- and hands each wavefront
+        // segments of similar length.
         if (P->nseg > 0) {
             const unsigned nbs = (unsigned)((P->nseg + 255) / 256);
             hipLaunchKernelGGL(k_iota_i32, dim3(nbs), dim3(256), 0, s, idx, P->nseg);
@@ -1109,10 +1114,10 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
             while ((1 << key_bits) < T.nkeys) ++key_bits;
             size_t sort_bytes = 0;
             if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, sk, sk2, idx, idx2, (int)P->nseg, 0,
-                                                   32 + key_bits, s) != hipSuccess ||
+                                                   38 + key_bits, s) != hipSuccess ||
                 hipMalloc(&sort_tmp, sort_bytes ? sort_bytes : 1) != hipSuccess ||
                 hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, sk, sk2, idx, idx2, (int)P->nseg, 0,
-                                                   32 + key_bits, s) != hipSuccess) {
+                                                   38 + key_bits, s) != hipSuccess) {
                 (void)hipGetLastError();
                 return fail(SRF_ERR_LAUNCH);
             }

@@ -100,9 +100,11 @@ class ScatterEstimator:
     identical to the plain sum on the surrogate's 4 mm training grid, where
     the scatter point cell is exactly 8 mm)."""
 
-    def __init__(self, geometry=None, net=None, kappa=None, check_rings=(16,), tol=0.02, fraction=0.3):
+    def __init__(self, geometry=None, net=None, kappa=None, check_rings=(16,), tol=0.02, fraction=0.3,
+                 coarse_spacing=None):
         self.g = geometry or SssGeometry()
         self.net, self.kappa = net, kappa
+        self.coarse_spacing = coarse_spacing or None
         self.check_rings, self.tol, self.fraction = tuple(check_rings), float(tol), float(fraction)
         self._sss, self._chk, self._feat = {}, {}, {}
         self.fallback = {}        # scan key -> True once a check failed
@@ -125,7 +127,8 @@ class ScatterEstimator:
         x_xyz = body_estimate(x_xyz, sc.meta["mu"])
         k = self._key(sc)
         if k not in self._feat:
-            self._feat[k] = SinogramFeatures(self.g, sc.shape, sc.center, sc.size, self.net.dev)
+            self._feat[k] = SinogramFeatures(self.g, sc.shape, sc.center, sc.size, self.net.dev,
+                                             coarse_spacing=self.coarse_spacing)
         feat = self._feat[k]
         xf, m = feat(x_xyz, sc.meta["mu"])
         out = self.net.forward(xf, feat.dims)

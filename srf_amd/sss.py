@@ -171,7 +171,7 @@ class SingleScatterSimulation:
     the output is then (len(rings), n_angles, n_radial)."""
 
     def __init__(self, geometry, shape, center, size, physical_kn=False, tiled=False, compact=True, rings=None,
-                 cell_normalized=False, physical_grid=False):
+                 cell_normalized=False, physical_grid=False, cell_ref=None):
         self.g = geometry
         self.shape = [int(v) for v in shape]
         self.center = [float(v) for v in center]
@@ -209,10 +209,11 @@ class SingleScatterSimulation:
                                                      geometry.resolution, physical_kn)).to(self.dev)
         self.scale = sss_scale(geometry.low_energy, geometry.high_energy, geometry.resolution)
         if cell_normalized:
-            # weight each point by its cell volume in units of a spacing^3 cell: the
-            # sum then approximates the same volume integral on every image grid
-            # (the factor is 1 where the point step times the voxel equals spacing)
-            self.scale *= self.cell / geometry.spacing ** 3
+            # weight each point by its cell volume in units of a reference cell
+            # (default spacing^3): the sum then approximates the same volume
+            # integral on every image grid and point spacing (the factor is 1 where
+            # the point step times the voxel equals the reference edge)
+            self.scale *= self.cell / (geometry.spacing ** 3 if cell_ref is None else float(cell_ref))
 
     def _lors(self, pts_xyz):
         ns = pts_xyz.shape[0]

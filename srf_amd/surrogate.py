@@ -261,10 +261,10 @@ class UNet:
                 check(lib().srf_cnn_pool_bwd(B[f"gp{l - 1}"].data_ptr(), pd[0], pd[1], pd[2], w[l - 1], w[l - 1],
                                              B[f"ge{l - 1}"].data_ptr(), w[l - 1], s), "srf_cnn_pool_bwd")
 
-    def loss_and_grad(self, out, target, dims):
-        """MSE of out vs target into the loss buffer; returns dL/dout (buffer)."""
+    def loss_and_grad(self, out, target, dims, weight=1.0):
+        """weight * MSE of out vs target into the loss buffer; returns dL/dout."""
         B = self._buffers(*dims)
-        check(lib().srf_cnn_mse(out.data_ptr(), target.data_ptr(), out.numel(), B["gout"].data_ptr(),
+        check(lib().srf_cnn_mse(out.data_ptr(), target.data_ptr(), out.numel(), float(weight), B["gout"].data_ptr(),
                                 B["loss"].data_ptr(), stream_handle(self.dev)), "srf_cnn_mse")
         return B["gout"]
 
@@ -293,12 +293,19 @@ def to_bf16(x):
     return y
 
 
+# Scale of the coarse-SSS input channel (about the unit-mean normalisation of
+# the targets, kappa, of the trained networks).
+COARSE_KAPPA = 8.0e10
+
+
 class SinogramFeatures:
     """Per-bin emission and attenuation line integrals along the coarse
     sinogram's LORs (the bins' detector pairs), assembled into the network's
-    8-channel bf16 input by srf_cnn_features."""
+    8-channel bf16 input by srf_cnn_features.  With coarse_spacing, channel 4
+    is an SSS on a coarse scatter-point grid (cell-volume normalised to the
+    fine grid's 8 mm cells), scaled like the targets."""
 
-    def __init__(self, geometry, shape, center, size, device=None):
+    def __init__(self, geometry, shape, center, size, device=None, coarse_spacing=None):
         self.g = geometry
         self.shape = [int(v) for v in shape]
         self.geom = [float(v) for v in center] + [float(v) for v in size]
@@ -311,6 +318,10 @@ class SinogramFeatures:
         self.to_integral = (L2 / (dx_ * dx_)).contiguous()
         self.dims = (geometry.n_rings, geometry.n_angles, geometry.n_radial)
         self.n = self.lors.shape[0]
+        self.coarse = None
+        if coarse_spacing:
+            self.coarse = SingleScatterSimulation(geometry.update(spacing=float(coarse_spacing)), shape, center, size,
+                                                  cell_normalized=True, cell_ref=geometry.spacing ** 3)
 
     def integrals(self, img_xyz):
         img_k = ops.to_kernel_layout(img_xyz.to(self.dev, torch.float32).contiguous())
@@ -321,10 +332,15 @@ class SinogramFeatures:
         em = self.integrals(em_xyz)
         mu = self.integrals(mu_xyz)
         m = em.mean().reshape(1).clamp_min(1e-30)
-        inv = (1.0 / m).contiguous()
+        scale = torch.cat([1.0 / m, COARSE_KAPPA / m]).contiguous()
+        extra = None
+        if self.coarse is not None:
+            extra = self.coarse(em_xyz.to(self.dev, torch.float32), mu_xyz.to(self.dev, torch.float32))
+            extra = extra.reshape(-1).contiguous()
         x = torch.empty((self.n, 8), dtype=torch.int16, device=self.dev)
-        check(lib().srf_cnn_features(em.data_ptr(), mu.data_ptr(), self.n, inv.data_ptr(), x.data_ptr(),
-                                     stream_handle(self.dev)), "srf_cnn_features")
+        check(lib().srf_cnn_features(em.data_ptr(), mu.data_ptr(), extra.data_ptr() if extra is not None else None,
+                                     self.n, scale.data_ptr(), x.data_ptr(), stream_handle(self.dev)),
+              "srf_cnn_features")
         return x, m
 
 
@@ -340,9 +356,9 @@ class ScatterSurrogate:
     the full SSS."""
 
     def __init__(self, net, kappa, geometry, shape, center, size, check_rings=(5, 16, 26), tol=0.02,
-                 physical_kn=False):
+                 physical_kn=False, coarse_spacing=None):
         self.net, self.kappa, self.g, self.tol = net, float(kappa), geometry, float(tol)
-        self.features = SinogramFeatures(geometry, shape, center, size, net.dev)
+        self.features = SinogramFeatures(geometry, shape, center, size, net.dev, coarse_spacing=coarse_spacing)
         self.sss_args = (geometry, shape, center, size)
         self.physical_kn = physical_kn
         self.check_rings = tuple(int(r) for r in check_rings) if check_rings else ()
@@ -375,11 +391,14 @@ class ScatterSurrogate:
         return pred, info
 
 
-def train(net, X, T, dims, epochs, batch=4, lr=1e-3, seed=0, log=None, lr_min_frac=0.05, time_limit=None):
+def train(net, X, T, dims, epochs, batch=4, lr=1e-3, seed=0, log=None, lr_min_frac=0.05, time_limit=None,
+          relative=False):
     """Adam on the MSE between net(X[i]) and T[i] (both on the GPU:
-    X (N, V, 8) int16 bf16, T (N, V) float32).  Cosine learning-rate decay to
-    lr * lr_min_frac.  Returns the per-epoch mean losses."""
+    X (N, V, 8) int16 bf16, T (N, V) float32), each sample's MSE divided by
+    mean(T[i]^2) with `relative` (the squared relative L2 error).  Cosine
+    learning-rate decay to lr * lr_min_frac.  Returns the per-epoch mean losses."""
     N = X.shape[0]
+    w = (1.0 / (T.double() ** 2).mean(1)).tolist() if relative else [1.0] * N
     g = torch.Generator().manual_seed(int(seed))
     B = net._buffers(*dims)
     steps_per_epoch = max(1, N // batch)
@@ -392,7 +411,7 @@ def train(net, X, T, dims, epochs, batch=4, lr=1e-3, seed=0, log=None, lr_min_fr
             net.zero_grad()
             for i in perm[sidx * batch:(sidx + 1) * batch].tolist():
                 out = net.forward(X[i], dims)
-                gout = net.loss_and_grad(out, T[i], dims)
+                gout = net.loss_and_grad(out, T[i], dims, w[i])
                 net.backward(gout, dims)
             frac = k / max(1, total - 1)
             cur = lr * (lr_min_frac + (1 - lr_min_frac) * 0.5 * (1 + math.cos(math.pi * frac)))

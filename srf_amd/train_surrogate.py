@@ -30,12 +30,12 @@ def _log(*a):
     print(*a, flush=True)
 
 
-def workload(device):
+def workload(device, coarse_spacing=None):
     g = SssGeometry()
     size = [SHAPE[0] * VOXEL, SHAPE[1] * VOXEL, SHAPE[2] * VOXEL]
     center = [0.0, 0.0, 0.0]
     sss = SingleScatterSimulation(g, SHAPE, center, size)
-    feat = SinogramFeatures(g, SHAPE, center, size, device)
+    feat = SinogramFeatures(g, SHAPE, center, size, device, coarse_spacing=coarse_spacing)
     return g, center, size, sss, feat
 
 
@@ -104,7 +104,7 @@ def osem_cases(seed, device, n_iter=3):
     return out
 
 
-def make_osem_pairs(n_pairs, seed0, sss_geometry, device, log=_log, n_iter=3):
+def make_osem_pairs(n_pairs, seed0, sss_geometry, device, log=_log, n_iter=3, coarse_spacing=None):
     """n_pairs in-loop pairs (n_iter per simulated scan): features, cell-
     normalised SSS sinograms and emission means, as make_pairs returns."""
     from .surrogate import SinogramFeatures as SF
@@ -118,7 +118,7 @@ def make_osem_pairs(n_pairs, seed0, sss_geometry, device, log=_log, n_iter=3):
             if i >= n_pairs:
                 break
             s_i = SingleScatterSimulation(sss_geometry, shape, center, size, cell_normalized=True)
-            f_i = SF(sss_geometry, shape, center, size, device)
+            f_i = SF(sss_geometry, shape, center, size, device, coarse_spacing=coarse_spacing)
             S[i] = s_i(em, mu).reshape(-1)
             x, m = f_i(em, mu)
             X[i] = x
@@ -131,7 +131,7 @@ def make_osem_pairs(n_pairs, seed0, sss_geometry, device, log=_log, n_iter=3):
     return X, S, M
 
 
-def make_pairs(seeds, sss, feat, device, log=_log, varied=None):
+def make_pairs(seeds, sss, feat, device, log=_log, varied=None, coarse_spacing=None):
     """Features X (n, V, 8) bf16 bits, SSS sinograms S (n, V), emission means
     M (n,).  varied: None (all on the surrogate grid) or a per-seed flag list
     selecting varied_case pairs, whose SSS is cell-volume normalised (equal
@@ -146,7 +146,7 @@ def make_pairs(seeds, sss, feat, device, log=_log, varied=None):
         if varied is not None and varied[i]:
             act, mu, shape, center, size = varied_case(int(seed), device)
             s_i = SingleScatterSimulation(g, shape, center, size, cell_normalized=True)
-            f_i = SinogramFeatures(g, shape, center, size, device)
+            f_i = SinogramFeatures(g, shape, center, size, device, coarse_spacing=coarse_spacing)
             S[i] = s_i(act, mu).reshape(-1)
             x, m = f_i(act, mu)
         else:
@@ -184,23 +184,33 @@ def main(argv=None):
     ap.add_argument("--in-loop", action="store_true",
                     help="train on OSEM estimates of simulated scans (osem_cases) instead of phantoms")
     ap.add_argument("--test-in-loop", type=int, default=0, help="held-out in-loop pairs")
+    ap.add_argument("--coarse-spacing", type=float, default=0.0,
+                    help="> 0: hybrid surrogate, an SSS on this scatter-point spacing (mm) as a fifth input")
+    ap.add_argument("--in-loop-fraction", type=float, default=1.0,
+                    help="with --in-loop: fraction of training pairs from OSEM estimates, the rest phantoms")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "weights",
                                                   "scatter_unet_128x128x64.npz"))
     a = ap.parse_args(argv)
     dev = torch.device("cuda", 0)
-    g, center, size, sss, feat = workload(dev)
+    cs = a.coarse_spacing or None
+    g, center, size, sss, feat = workload(dev, cs)
     dims = feat.dims
     _log(f"generating {a.train} training and {a.test} held-out pairs")
     rng = np.random.default_rng(a.seed + 17)
     flags = list(rng.uniform(size=a.train) < a.varied) if a.varied > 0 else None
     if a.in_loop:
-        Xtr, Str, Mtr = make_osem_pairs(a.train, 0, g, dev)
+        n_loop = int(round(a.train * a.in_loop_fraction))
+        Xtr, Str, Mtr = make_osem_pairs(n_loop, 0, g, dev, coarse_spacing=cs)
+        if n_loop < a.train:
+            X2, S2, M2 = make_pairs(range(a.train - n_loop), sss, feat, dev, varied=flags, coarse_spacing=cs)
+            Xtr, Str, Mtr = torch.cat([Xtr, X2]), torch.cat([Str, S2]), torch.cat([Mtr, M2])
+            del X2, S2, M2
     else:
-        Xtr, Str, Mtr = make_pairs(range(a.train), sss, feat, dev, varied=flags)
-    Xte, Ste, Mte = make_pairs(range(HELD_OUT_SEED0, HELD_OUT_SEED0 + a.test), sss, feat, dev)
+        Xtr, Str, Mtr = make_pairs(range(a.train), sss, feat, dev, varied=flags, coarse_spacing=cs)
+    Xte, Ste, Mte = make_pairs(range(HELD_OUT_SEED0, HELD_OUT_SEED0 + a.test), sss, feat, dev, coarse_spacing=cs)
     Xtl = None
     if a.test_in_loop:
-        Xtl, Stl, Mtl = make_osem_pairs(a.test_in_loop, HELD_OUT_SEED0, g, dev)
+        Xtl, Stl, Mtl = make_osem_pairs(a.test_in_loop, HELD_OUT_SEED0, g, dev, coarse_spacing=cs)
     Xtv = None
     if a.test_varied:
         vs = range(HELD_OUT_SEED0 + 500_000, HELD_OUT_SEED0 + 500_000 + a.test_varied)
@@ -208,10 +218,10 @@ def main(argv=None):
     # kappa puts the normalised targets at unit mean
     kappa = float(1.0 / (Str.mean(1) / Mtr).mean())
     Ttr = (Str / Mtr[:, None] * kappa).contiguous()
-    net = UNet(seed=a.seed, device=dev)
+    net = UNet(in_ch=5 if cs else 4, seed=a.seed, device=dev)
     t0 = time.time()
     hist = train(net, Xtr, Ttr, dims, a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=_log,
-                 time_limit=a.time_limit)
+                 time_limit=a.time_limit, relative=bool(cs))
     t_train = time.time() - t0
     err_tr = evaluate(net, Xtr[:200], Str[:200], Mtr[:200], kappa, dims)
     err = evaluate(net, Xte, Ste, Mte, kappa, dims)
@@ -233,7 +243,7 @@ def main(argv=None):
                      in_loop_max=float(el.max()), in_loop_frac_le_2pct=float((el <= 0.02).mean()))
     # timing: surrogate (features + network) vs SSS on one held-out phantom
     act, mu, _, _ = phantom.scatter_phantom(SHAPE, VOXEL, HELD_OUT_SEED0, device=dev)
-    sur = ScatterSurrogate(net, kappa, g, SHAPE, center, size, check_rings=())
+    sur = ScatterSurrogate(net, kappa, g, SHAPE, center, size, check_rings=(), coarse_spacing=cs)
     for _ in range(2):
         sur.predict(act, mu)
         sss(act, mu)
@@ -254,7 +264,7 @@ def main(argv=None):
              epochs=len(hist), batch=a.batch, lr=a.lr, train_seconds=round(t_train, 1),
              final_loss=float(hist[-1]), heldout_mean=float(err.mean()), heldout_max=float(err.max()),
              heldout_p95=float(np.percentile(err, 95)), varied_train_fraction=a.varied, in_loop=bool(a.in_loop),
-             **extra)
+             in_loop_fraction=a.in_loop_fraction if a.in_loop else 0.0, coarse_spacing=cs or 0.0, **extra)
     _log(f"saved {a.out}")
 
 

@@ -170,11 +170,11 @@ def test_pool_upsample_copy_act_bias_mse_adam_cast():
     t = torch.from_numpy(rng.normal(size=5000).astype(np.float32)).to(DEV)
     grad = torch.zeros(5000, device=DEV)
     loss = torch.zeros(1, device=DEV)
-    assert _L().srf_cnn_mse(p.data_ptr(), t.data_ptr(), 5000, grad.data_ptr(), loss.data_ptr(), _s()) == 0
+    assert _L().srf_cnn_mse(p.data_ptr(), t.data_ptr(), 5000, 0.5, grad.data_ptr(), loss.data_ptr(), _s()) == 0
     torch.cuda.synchronize()
     l_want, g_want = co.mse(p.double().cpu().numpy(), t.double().cpu().numpy())
-    assert float(loss) == pytest.approx(l_want, rel=1e-5)
-    np.testing.assert_allclose(grad.double().cpu().numpy(), g_want, rtol=1e-5, atol=1e-12)
+    assert float(loss) == pytest.approx(0.5 * l_want, rel=1e-5)
+    np.testing.assert_allclose(grad.double().cpu().numpy(), 0.5 * g_want, rtol=1e-5, atol=1e-12)
     # Adam, three steps, with gscale
     w = rng.normal(size=1000)
     wd_ = torch.from_numpy(w.astype(np.float32)).to(DEV)
