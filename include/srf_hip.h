@@ -124,6 +124,14 @@ int srf_lor_digest(const float *lors, int64_t n, int64_t lor_stride,
                    srf_grid_t grid, const srf_tof_t *tof, srf_bp_mode_t mode,
                    uint64_t *hash, int32_t *count, void *stream);
 
+/* Separable Gaussian smoothing of a C-order (n0, n1, n2) image (sigmas in
+ * voxels, <= 64; weights renormalised where the kernel leaves the image), then
+ * out *= mask (mask may be NULL).  x, tmp and out are distinct buffers of
+ * n0*n1*n2 floats.  The scatter estimate's emission input in the OSEM pipeline
+ * (no counterpart in the reference). */
+int srf_smooth(const float *x, int32_t n0, int32_t n1, int32_t n2, float sigma0, float sigma1, float sigma2,
+               const float *mask, float *tmp, float *out, void *stream);
+
 /* Layout conversions between the Python (nx,ny,nz) C-order image (z fastest)
  * and the kernel layout (x fastest): dst = transpose(src). */
 int srf_image_to_kernel_layout(const float *src_xyz, float *dst_kernel, int32_t nx, int32_t ny,

@@ -109,6 +109,30 @@ static int swap_outer(const float *src, float *dst, int R, int Y, int C, hipStre
     return launch_status();
 }
 
+// One pass of a separable Gaussian along an axis of a C-order (n0, n1, n2)
+// image: out[i] = sum_k w_k x[i + k*stride] / sum_k w_k over the taps inside
+// the image (weights renormalised at the edges, so a constant stays constant),
+// w_k = exp(-k^2 / (2 sigma^2)), |k| <= radius.  Optional mask: out *= mask.
+__global__ __launch_bounds__(256) void k_gauss_pass(const float *__restrict__ x, int64_t n, int len, int64_t stride,
+                                                    float sigma, int radius, const float *__restrict__ mask,
+                                                    float *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int pos = (int)((i / stride) % len);
+    const float c = -0.5f / (sigma * sigma);
+    float s = 0.0f, ws = 0.0f;
+    const int k0 = max(-radius, -pos), k1 = min(radius, len - 1 - pos);
+    for (int k = k0; k <= k1; ++k) {
+        const float w = expf(c * (float)(k * k));
+        s += w * x[i + (int64_t)k * stride];
+        ws += w;
+    }
+    float v = s / ws;
+    if (mask) v *= mask[i];
+    out[i] = v;
+}
+
 }  // namespace srf
 
 using namespace srf;
@@ -150,6 +174,27 @@ int srf_em_update(float *x, const float *bp, const float *emap, int64_t nvox, fl
         return SRF_ERR_INVALID_ARG;
     hipLaunchKernelGGL(k_em_update, dim3((unsigned)((nvox + 1023) / 1024)), dim3(256), 0, (hipStream_t)stream, x, bp,
                        emap, nvox, zero_guard);
+    return launch_status();
+}
+
+int srf_smooth(const float *x, int32_t n0, int32_t n1, int32_t n2, float sigma0, float sigma1, float sigma2,
+               const float *mask, float *tmp, float *out, void *stream)
+{
+    if (!x || !tmp || !out || n0 <= 0 || n1 <= 0 || n2 <= 0) return SRF_ERR_INVALID_ARG;
+    if (!(sigma0 > 0.0f) || !(sigma1 > 0.0f) || !(sigma2 > 0.0f)) return SRF_ERR_INVALID_ARG;
+    if (sigma0 > 64.0f || sigma1 > 64.0f || sigma2 > 64.0f) return SRF_ERR_UNSUPPORTED;
+    if (tmp == x || tmp == out || out == x) return SRF_ERR_INVALID_ARG;
+    const int64_t n = (int64_t)n0 * n1 * n2;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipStream_t s = (hipStream_t)stream;
+    auto rad = [](float sg) { return (int)ceilf(3.0f * sg); };
+    // axis 2 (contiguous) x -> out, axis 1 out -> tmp, axis 0 tmp -> out (masked)
+    hipLaunchKernelGGL(k_gauss_pass, dim3(nb), dim3(256), 0, s, x, n, n2, (int64_t)1, sigma2, rad(sigma2),
+                       (const float *)nullptr, out);
+    hipLaunchKernelGGL(k_gauss_pass, dim3(nb), dim3(256), 0, s, (const float *)out, n, n1, (int64_t)n2, sigma1,
+                       rad(sigma1), (const float *)nullptr, tmp);
+    hipLaunchKernelGGL(k_gauss_pass, dim3(nb), dim3(256), 0, s, (const float *)tmp, n, n0, (int64_t)n1 * n2,
+                       sigma0, rad(sigma0), mask, out);
     return launch_status();
 }
 

@@ -86,10 +86,29 @@ def make_scans(n_scans=32, seed=5, ev_range=(1e5, 2e7), sizes=(96, 128, 192, 256
     return scans
 
 
-def body_estimate(x_xyz, mu_xyz):
-    """The emission estimate restricted to the body (mu > 0): activity outside
-    tissue is unphysical, and early OSEM iterates spread some there."""
-    return x_xyz * (mu_xyz > 0).to(x_xyz.dtype)
+# Gaussian FWHM (mm) of the emission estimate that scatter is computed from.
+SCATTER_SMOOTH_MM = 20.0
+
+
+def body_estimate(x_xyz, mu_xyz, voxel=None, fwhm=SCATTER_SMOOTH_MM):
+    """The emission estimate that scatter is computed from: restricted to the
+    body (mu > 0; activity outside tissue is unphysical, and early OSEM iterates
+    spread some there) and, with `voxel` (mm), smoothed with a Gaussian of
+    `fwhm` mm (srf_smooth, weights renormalised at the image edge) -- early
+    iterates hold isolated hot voxels that scatter, a smooth functional of the
+    activity, does not resolve."""
+    from ._lib import check, lib, stream_handle
+    body = (mu_xyz > 0).to(torch.float32).contiguous()
+    x = (x_xyz.to(torch.float32) * body).contiguous()
+    if voxel is None or not fwhm:
+        return x
+    sig = float(fwhm) / 2.3548200450309493 / float(voxel)
+    tmp = torch.empty_like(x)
+    out = torch.empty_like(x)
+    n0, n1, n2 = x.shape
+    check(lib().srf_smooth(x.data_ptr(), n0, n1, n2, sig, sig, sig, body.data_ptr(), tmp.data_ptr(), out.data_ptr(),
+                           stream_handle(x.device)), "srf_smooth")
+    return out
 
 
 class ScatterEstimator:
@@ -115,7 +134,10 @@ class ScatterEstimator:
         return (tuple(sc.shape), tuple(sc.center), tuple(sc.size))
 
     def sss(self, sc, x_xyz):
-        x_xyz = body_estimate(x_xyz, sc.meta["mu"])
+        return self._sss_of(sc, body_estimate(x_xyz, sc.meta["mu"], sc.meta.get("voxel")))
+
+    def _sss_of(self, sc, x_xyz):
+        """SSS of an already regularised estimate."""
         k = self._key(sc)
         if k not in self._sss:
             self._sss[k] = SingleScatterSimulation(self.g, sc.shape, sc.center, sc.size, cell_normalized=True)
@@ -123,8 +145,8 @@ class ScatterEstimator:
 
     def surrogate(self, sc, x_xyz, check, scan_id):
         if self.fallback.get(scan_id):
-            return self.sss(sc, x_xyz), True
-        x_xyz = body_estimate(x_xyz, sc.meta["mu"])
+            return self._sss_of(sc, body_estimate(x_xyz, sc.meta["mu"], sc.meta.get("voxel"))), True
+        x_xyz = body_estimate(x_xyz, sc.meta["mu"], sc.meta.get("voxel"))
         k = self._key(sc)
         if k not in self._feat:
             self._feat[k] = SinogramFeatures(self.g, sc.shape, sc.center, sc.size, self.net.dev,
@@ -142,7 +164,7 @@ class ScatterEstimator:
             self.checks.append((scan_id, err))
             if not err <= self.tol:
                 self.fallback[scan_id] = True
-                return self.sss(sc, x_xyz), True
+                return self._sss_of(sc, x_xyz), True
         return pred, False
 
     def values(self, sc, sino):

@@ -99,7 +99,7 @@ def osem_cases(seed, device, n_iter=3):
     out = []
     for _ in range(n_iter):
         b.iteration()
-        out.append((body_estimate(b.image(0), mu), mu, shape, center, size))
+        out.append((body_estimate(b.image(0), mu, vox), mu, shape, center, size))
     b.close()
     return out
 
