@@ -319,14 +319,14 @@ __device__ __forceinline__ int cvt_flr(float v)
 
 enum : int { M_GATHER = 0, M_SCATTER = 1, M_DIGEST = 2, M_SCATTER_FX = 3 };
 
-// c * 2^shift as a 64-bit two's complement integer, exactly split into 32-bit
-// halves in float arithmetic (|c * 2^shift| < 2^49 by the choice of shift):
-// hi = floor(cs / 2^32) is an integer-valued float, lo = cs - hi * 2^32 lies in
-// [0, 2^32) and is exact (it keeps cs's 24-bit mantissa); the fraction below one
-// unit (2^-shift, 2^-49 of the item's largest possible term) is truncated.
-__device__ __forceinline__ unsigned long long to_fixed(float c, int shift)
+// A term already scaled by the item's 2^shift (|cs| < 2^49 by the choice of
+// shift) as a 64-bit two's complement integer, exactly split into 32-bit
+// halves in float arithmetic: hi = floor(cs / 2^32) is an integer-valued
+// float, lo = cs - hi * 2^32 lies in [0, 2^32) and is exact (it keeps cs's
+// 24-bit mantissa); the fraction below one unit (2^-49 of the item's largest
+// possible term) is truncated.
+__device__ __forceinline__ unsigned long long to_fixed_scaled(float cs)
 {
-    const float cs = ldexpf(c, shift);
     const float hi = floorf(cs * 2.3283064365386963e-10f);
     const float lo = fmaf(-hi, 4294967296.0f, cs);
     const unsigned long long uh = (unsigned long long)(unsigned)(int)hi;
@@ -371,7 +371,13 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const float zv = z1 + kz * (pp1 - p1);
         Z0 = (DZ ? zv / G.dz : zv) + G.nz2;
     }
-    int base = (a - t.p0) * t.sp;
+    // LDS addresses in bytes (element size ES): no per-term index scaling
+    constexpr int ES = (MODE == M_SCATTER_FX) ? 8 : 4;
+    int base = (a - t.p0) * t.sp * ES;
+    const int bstep = t.sp * ES;
+    const int sstep_log2 = (t.ss == 1 ? 0 : 5) + (ES == 8 ? 3 : 2);   // secondary stride 1 or TX = 32
+    const int dummy = t.dummy * ES;
+    static_assert(TX == 32 && TY == 32, "secondary stride is 1 or 32");
     for (int ip = a; ip < b; ++ip) {
         const float pp2 = pp1 + 1.0f;
         float S1, Z1;
@@ -413,33 +419,42 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const bool v1 = ok && cs1 >= 0 && cz1 >= 0;
         const bool v2 = both & (sgt ? ((cs1 >= 0) & (cz2 < nz)) : ((cs2 < ns) & (cz1 >= 0)));
         const bool v3 = ok && (bs || bz) && cs2 < ns && cz2 < nz;
-        const int a1 = cs1 - t.s0, b1 = cz1 - t.z0;
-        const int a2 = c2s - t.s0, b2 = c2z - t.z0;
-        const int a3 = cs2 - t.s0, b3 = cz2 - t.z0;
-        const int TS = xdom ? TY : TX;
-        const bool in1 = v1 && (unsigned)a1 < (unsigned)TS && (unsigned)b1 < (unsigned)TZ;
-        const bool in2 = v2 && (unsigned)a2 < (unsigned)TS && (unsigned)b2 < (unsigned)TZ;
-        const bool in3 = v3 && (unsigned)a3 < (unsigned)TS && (unsigned)b3 < (unsigned)TZ;
-        const int i1 = in1 ? base + a1 * t.ss + b1 * (TX * TY) : t.dummy;
-        const int i2 = in2 ? base + a2 * t.ss + b2 * (TX * TY) : t.dummy;
-        const int i3 = in3 ? base + a3 * t.ss + b3 * (TX * TY) : t.dummy;
+        // the two secondary and two axial cells relative to the tile: term 1 is
+        // (cs1, cz1), term 3 (cs2, cz2), term 2 (cs1, cz2) or (cs2, cz1)
+        const int rs1 = cs1 - t.s0, rs2 = cs2 - t.s0, rz1 = cz1 - t.z0, rz2 = cz2 - t.z0;
+        const bool ts1 = (unsigned)rs1 < (unsigned)TX, ts2 = (unsigned)rs2 < (unsigned)TX;
+        const bool tz1 = (unsigned)rz1 < (unsigned)TZ, tz2 = (unsigned)rz2 < (unsigned)TZ;
+        const int os1 = (int)((unsigned)rs1 << sstep_log2), os2 = (int)((unsigned)rs2 << sstep_log2);
+        const int oz1 = rz1 * (TX * TY * ES), oz2 = rz2 * (TX * TY * ES);
+        const bool in1 = v1 && ts1 && tz1;
+        const bool in2 = v2 && (sgt ? (ts1 && tz2) : (ts2 && tz1));
+        const bool in3 = v3 && ts2 && tz2;
+        const int i1 = in1 ? base + os1 + oz1 : dummy;
+        const int i2 = in2 ? base + (sgt ? os1 + oz2 : os2 + oz1) : dummy;
+        const int i3 = in3 ? base + os2 + oz2 : dummy;
         if (MODE == M_GATHER) {
-            const float g1 = t.lds[i1], g2 = t.lds[i2], g3 = t.lds[i3];
-            t.acc += g1 * (r1 * wg);
-            t.acc += g2 * (r2 * wg);
-            t.acc += g3 * (r3 * wg);
+            const char *L = reinterpret_cast<const char *>(t.lds);
+            const float g1 = *reinterpret_cast<const float *>(L + i1);
+            const float g2 = *reinterpret_cast<const float *>(L + i2);
+            const float g3 = *reinterpret_cast<const float *>(L + i3);
+            t.acc = fmaf(g1, r1 * wg, t.acc);
+            t.acc = fmaf(g2, r2 * wg, t.acc);
+            t.acc = fmaf(g3, r3 * wg, t.acc);
         } else if (MODE == M_SCATTER) {
             // float LDS atomics (non-finite values only; see k_bp_tiled)
+            char *L = reinterpret_cast<char *>(t.lds);
             const float vw = t.value * wg;
-            if (in1) atomicAdd(t.lds + i1, vw * r1);
-            if (in2) atomicAdd(t.lds + i2, vw * r2);
-            if (in3) atomicAdd(t.lds + i3, vw * r3);
+            if (in1) atomicAdd(reinterpret_cast<float *>(L + i1), vw * r1);
+            if (in2) atomicAdd(reinterpret_cast<float *>(L + i2), vw * r2);
+            if (in3) atomicAdd(reinterpret_cast<float *>(L + i3), vw * r3);
         } else if (MODE == M_SCATTER_FX) {
-            // 64-bit fixed point: integer LDS atomics run at full rate
+            // 64-bit fixed point: integer LDS atomics run at full rate; t.value
+            // already carries the item's scale 2^shift
+            char *L = reinterpret_cast<char *>(t.fx);
             const float vw = t.value * wg;
-            if (in1) atomicAdd(t.fx + i1, to_fixed(vw * r1, t.shift));
-            if (in2) atomicAdd(t.fx + i2, to_fixed(vw * r2, t.shift));
-            if (in3) atomicAdd(t.fx + i3, to_fixed(vw * r3, t.shift));
+            if (in1) atomicAdd(reinterpret_cast<unsigned long long *>(L + i1), to_fixed_scaled(vw * r1));
+            if (in2) atomicAdd(reinterpret_cast<unsigned long long *>(L + i2), to_fixed_scaled(vw * r2));
+            if (in3) atomicAdd(reinterpret_cast<unsigned long long *>(L + i3), to_fixed_scaled(vw * r3));
         } else {
             const int pg = ip * t.gsp;
             const int bb1 = bs ? (bz ? (sgt ? 5 : 8) : 3) : (bz ? 1 : 0);
@@ -460,7 +475,7 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         pp1 = pp2;
         S0 = S1;
         Z0 = Z1;
-        base += t.sp;
+        base += bstep;
     }
 }
 
@@ -608,7 +623,7 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         walk_for(t, fl, TVOX, (R.meta & 1u) != 0, x0, y0, z0);
         t.fx = fx;
         t.shift = shift;
-        t.value = vv[k];
+        t.value = fixed ? ldexpf(vv[k], shift) : vv[k];   // scale folded into the value
         if (fixed) march_tile<TOF, DS, DZ, M_SCATTER_FX>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
         else march_tile<TOF, DS, DZ, M_SCATTER>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
     }
