@@ -401,7 +401,9 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         float rs = bs ? ns_ * __builtin_amdgcn_rcpf(dsv) : 1.0f;
         float rz = bz ? nz_ * __builtin_amdgcn_rcpf(dzv) : 1.0f;
         const bool both = ok && bs && bz;
-        if (both && !(fabsf(rs - rz) > 4e-6f * fmaxf(rs, rz))) {
+        // rs, rz lie in (0, 1]: an absolute margin is at least as strict as the
+        // relative one the estimates' few-ulp error needs
+        if (both && !(fabsf(rs - rz) > 4e-6f)) {
             rs = ns_ / dsv;
             rz = nz_ / dzv;
         }
@@ -416,8 +418,10 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const float r2 = sgt ? (rs - rz) : (rz - rs);
         const float r3 = bs ? (bz ? (sgt ? 1.0f - rs : 1.0f - rz) : 1.0f - rs) : 1.0f - rz;
         const int c2s = sgt ? cs1 : cs2, c2z = sgt ? cz2 : cz1;
-        const bool v1 = ok && cs1 >= 0 && cz1 >= 0;
-        const bool v2 = both & (sgt ? ((cs1 >= 0) & (cz2 < nz)) : ((cs2 < ns) & (cz1 >= 0)));
+        // the reference's validity tests; `c >= 0` parts are implied by the
+        // in-tile tests below (tile origins are >= 0) and left to them
+        const bool v1 = ok;
+        const bool v2 = both & (sgt ? (cz2 < nz) : (cs2 < ns));
         const bool v3 = ok && (bs || bz) && cs2 < ns && cz2 < nz;
         // the two secondary and two axial cells relative to the tile: term 1 is
         // (cs1, cz1), term 3 (cs2, cz2), term 2 (cs1, cz2) or (cs2, cz1)
@@ -437,9 +441,7 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
             const float g1 = *reinterpret_cast<const float *>(L + i1);
             const float g2 = *reinterpret_cast<const float *>(L + i2);
             const float g3 = *reinterpret_cast<const float *>(L + i3);
-            t.acc = fmaf(g1, r1 * wg, t.acc);
-            t.acc = fmaf(g2, r2 * wg, t.acc);
-            t.acc = fmaf(g3, r3 * wg, t.acc);
+            t.acc = fmaf(fmaf(g1, r1, fmaf(g2, r2, g3 * r3)), wg, t.acc);
         } else if (MODE == M_SCATTER) {
             // float LDS atomics (non-finite values only; see k_bp_tiled)
             char *L = reinterpret_cast<char *>(t.lds);
