@@ -41,9 +41,10 @@ FWHM = 60.0
 # Image grids may extend past the scanner's axial length, where the sensitivity
 # image is 0: those voxels take the guard of mlem.py:33-35 instead of 0/0.
 ZERO_GUARD = 1e8
-# TOF window of the plans (srf_plan_opts_t.tof_window): terms below 2^-26 of
-# the event's TOF peak are not binned (< 2e-9 of its weight).
-TOF_WINDOW = 26.0
+# TOF window of the plans (srf_plan_opts_t.tof_window): terms below 2^-22 of
+# the event's TOF peak are not binned (< 4e-8 of its weight: below one float32
+# ulp of the event's total).
+TOF_WINDOW = 22.0
 
 
 def _log(*a):

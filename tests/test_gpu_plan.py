@@ -150,9 +150,9 @@ def test_plan_project_ratio_fused():
 
 
 def test_tof_window_drops_only_negligible_terms():
-    """srf_plan_opts_t.tof_window = 26 at the OSEM workload's geometry: about a
+    """srf_plan_opts_t.tof_window = 22 at the OSEM workload's geometry: about a
     third fewer terms, projections and backprojections within 1e-6 of the
-    exact plan (the dropped TOF weight is < 2e-9 of each event's)."""
+    exact plan (the dropped TOF weight is < 4e-8 of each event's)."""
     import torch
     from srf_amd import ops, phantom
     shape, voxel = (256, 256, 128), 2.0
@@ -161,7 +161,7 @@ def test_tof_window_drops_only_negligible_terms():
     lors = phantom.sample_events(act, center, size, 200000, 424.0, 300.0, 6, tof_fwhm_mm=60.0, device="cuda")
     geom = list(center) + list(size)
     exact = ops.Plan(lors, shape, geom, True, 3.66, s2)
-    win = ops.Plan(lors, shape, geom, True, 3.66, s2, tof_window=26.0)
+    win = ops.Plan(lors, shape, geom, True, 3.66, s2, tof_window=22.0)
     for mode in (1, 0):
         _, c0 = exact.digest(mode)
         _, c1 = win.digest(mode)
