@@ -319,39 +319,62 @@ __device__ __forceinline__ int cvt_flr(float v)
 
 enum : int { M_GATHER = 0, M_SCATTER = 1, M_DIGEST = 2, M_SCATTER_FX = 3 };
 
-// A term already scaled by the item's 2^shift (|cs| < 2^49 by the choice of
-// shift) as a 64-bit two's complement integer, exactly split into 32-bit
-// halves in float arithmetic: hi = floor(cs / 2^32) is an integer-valued
-// float, lo = cs - hi * 2^32 lies in [0, 2^32) and is exact (it keeps cs's
-// 24-bit mantissa); the fraction below one unit (2^-49 of the item's largest
-// possible term) is truncated.
-__device__ __forceinline__ unsigned long long to_fixed_scaled(float cs)
+// A term v * r (v already scaled by the item's 2^shift, |v * r| < 2^49 by the
+// choice of shift) as a 64-bit two's complement integer, rounded to nearest.
+// The float product is exact in double; adding 1.5 * 2^52 in the same fma
+// rounds it to an integer (the sum's unit spacing is 1), and the sum's bit
+// pattern is then 0x4338000000000000 + round(v * r) for |v * r| < 2^51.  The
+// constant's low word is zero, so the subtraction touches the high word only.
+__device__ __forceinline__ unsigned long long fixed_term(double v, float r)
 {
-    const float hi = floorf(cs * 2.3283064365386963e-10f);
-    const float lo = fmaf(-hi, 4294967296.0f, cs);
-    const unsigned long long uh = (unsigned long long)(unsigned)(int)hi;
-    return (uh << 32) | (unsigned long long)(unsigned)lo;
+    const double s = fma(v, (double)r, 6755399441055744.0);
+    return (unsigned long long)__double_as_longlong(s) - 0x4338000000000000ull;
+}
+
+// Minimum / maximum of two non-negative finite floats as integer operations on
+// their bit patterns (which order the same way): no NaN canonicalisation.
+__device__ __forceinline__ float min_nonneg(float a, float b)
+{
+    return __uint_as_float(min(__float_as_uint(a), __float_as_uint(b)));
+}
+__device__ __forceinline__ float max_nonneg(float a, float b)
+{
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
 }
 
 // State of one segment walk inside one LDS tile (stored x fastest:
 // lds[lx + TX*(ly + TY*lz)]).  Terms whose cell lies outside the tile, or that
 // the reference's predicates reject, are sent to `dummy`: a zero word for
-// gathers, a per-lane scratch word for scatters, so the slab body has no
-// branches and its three LDS accesses issue back to back.
+// gathers (scatters skip them), so the gather body has no branches and its
+// three LDS reads issue back to back.
 struct TileWalk {
     float *lds;
     unsigned long long *fx;  // fixed-point accumulator tile
-    int shift;
     int dummy;
     int p0, s0, z0;    // tile origin along primary, secondary, axial
     int sp, ss;        // LDS strides of primary / secondary
-    float acc;         // gather accumulator
-    float value;       // scatter value
+    float acc;         // gather result (times the segment's weight constant)
+    float value;       // scatter value (for fixed point: times 2^shift)
     uint64_t h;        // digest
     int cnt;
     int gsp, gss, gsz; // global strides (digest)
 };
 
+// One segment: slabs [a, b) of record R inside the walk's tile.
+//
+// Per slab this evaluates the reference's cell indices and predicates exactly
+// (siddon.cu.cc:86-189 for x-dominant rays, 231-333 for y-dominant; TOF
+// siddon_tof.cu.cc:102-210, 244-366).  With cs1 <= cs2 the two secondary cells
+// a slab spans and cz1 <= cz2 the axial ones, the reference's top-level test
+// is cs1 >= (cs1 != cs2 ? -1 : 0) && cs1 < ns (and the same along z), term 2
+// also needs its own cell below ns / nz, and term 3 both.  Restricted to the
+// cells of the tile that lie inside the image (0 <= c - origin < extent, the
+// extent cut at the image edge), every upper bound and every `>= 0` part is
+// implied by the tile test of the term's own cell, which leaves:
+//   term 1 (cs1, cz1): its tile test;
+//   term 2 (cs1, cz2) if rs > rz, else (cs2, cz1): both crossings, its tile
+//          test, and cz1 >= -1 (resp. cs1 >= -1);
+//   term 3 (cs2, cz2): a crossing, its tile test, cs1 >= -1 and cz1 >= -1.
 template <bool TOF, bool DS, bool DZ, int MODE>
 __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G, int a, int b, float W,
                                            TileWalk &t)
@@ -363,6 +386,8 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
     const float np2 = xdom ? G.nx2 : G.ny2;
     const int ns = xdom ? G.ny : G.nx, nz = G.nz;
     const bool ks_pos = ks >= 0.0f, kz_pos = kz >= 0.0f;
+    // in-image extent of the tile along secondary / axial
+    const unsigned es = (unsigned)min(TX, ns - t.s0), ez = (unsigned)min(TZ, nz - t.z0);
     float pp1 = (float)a - np2;
     float S0, Z0;
     {
@@ -371,14 +396,21 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const float zv = z1 + kz * (pp1 - p1);
         Z0 = (DZ ? zv / G.dz : zv) + G.nz2;
     }
-    // LDS addresses in bytes (element size ES): no per-term index scaling
+    // LDS byte address of cell (c_s, c_z) of slab ip: base + (c_s << sstep_log2)
+    // + (c_z << ZSH), the tile origin folded into base
     constexpr int ES = (MODE == M_SCATTER_FX) ? 8 : 4;
-    int base = (a - t.p0) * t.sp * ES;
+    constexpr int ZSH = (ES == 8 ? 3 : 2) + 10;                           // TX * TY = 1024 cells per layer
+    const int sstep_log2 = (t.ss == 1 ? 0 : 5) + (ES == 8 ? 3 : 2);       // secondary stride 1 or TX = 32
+    int base = (a - t.p0) * t.sp * ES - (t.s0 << sstep_log2) - (t.z0 << ZSH);
     const int bstep = t.sp * ES;
-    const int sstep_log2 = (t.ss == 1 ? 0 : 5) + (ES == 8 ? 3 : 2);   // secondary stride 1 or TX = 32
     const int dummy = t.dummy * ES;
     static_assert(TX == 32 && TY == 32, "secondary stride is 1 or 32");
-    for (int ip = a; ip < b; ++ip) {
+    float acc = 0.0f;
+    const float vw = t.value * W;
+    const double vwd = (double)vw;
+    const int base_end = base + (b - a) * bstep;
+    int ip = a;   // digest only
+    for (; base < base_end; base += bstep) {
         const float pp2 = pp1 + 1.0f;
         float S1, Z1;
         {
@@ -392,93 +424,98 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const int cs1 = cvt_flr(sl), cs2 = cvt_flr(sh);
         const int cz1 = cvt_flr(zl), cz2 = cvt_flr(zh);
         const bool bs = cs1 != cs2, bz = cz1 != cz2;
-        // top-level tests of siddon.cu.cc:86-189 (x) / 231-333 (y)
-        const bool ok = cs1 >= (bs ? -1 : 0) && cs1 < ns && cz1 >= (bz ? -1 : 0) && cz1 < nz;
+        // LDS addresses of the three terms (dummy where the term is not
+        // applied); term 2 is i2a if rs > rz, else i2b
+        const bool ts1 = (unsigned)(cs1 - t.s0) < es, ts2 = (unsigned)(cs2 - t.s0) < es;
+        const bool tz1 = (unsigned)(cz1 - t.z0) < ez, tz2 = (unsigned)(cz2 - t.z0) < ez;
+        const bool los = cs1 >= -1, loz = cz1 >= -1;
+        const int A = base + (cs1 << sstep_log2), B = base + (cs2 << sstep_log2);
+        // (logical operators keep the predicates as lane masks)
+        const int i1 = (ts1 && tz1) ? A + (cz1 << ZSH) : dummy;
+        const int i3 = ((bs || bz) && ts2 && tz2 && los && loz) ? B + (cz2 << ZSH) : dummy;
+        const int i2a = (bs && bz && ts1 && tz2 && loz) ? A + (cz2 << ZSH) : dummy;
+        const int i2b = (bs && bz && ts2 && tz1 && los) ? B + (cz1 << ZSH) : dummy;
+        // materialise the addresses here: sunk past the branch below, their
+        // lane-mask operands would be carried across it as 0/1 vectors
+        asm volatile("" ::"v"(i1), "v"(i2a), "v"(i2b), "v"(i3));
         // crossing fractions: reciprocal estimates, exact quotients only where
-        // the estimate cannot decide rs > rz (which picks the middle voxel)
+        // the estimate cannot decide rs > rz (which picks the middle voxel);
+        // rs = 1 (rs0 = 0) along an axis that is not crossed
         const float dsv = sh - sl, dzv = zh - zl;
         const float ns_ = (float)cs2 - sl, nz_ = (float)cz2 - zl;
-        float rs = bs ? ns_ * __builtin_amdgcn_rcpf(dsv) : 1.0f;
-        float rz = bz ? nz_ * __builtin_amdgcn_rcpf(dzv) : 1.0f;
-        const bool both = ok && bs && bz;
+        const float es_ = ns_ * __builtin_amdgcn_rcpf(dsv), ez_ = nz_ * __builtin_amdgcn_rcpf(dzv);
+        float rs = bs ? es_ : 1.0f, rz = bz ? ez_ : 1.0f;
+        float rs0 = bs ? es_ : 0.0f, rz0 = bz ? ez_ : 0.0f;
+        float dr = rs - rz;
         // rs, rz lie in (0, 1]: an absolute margin is at least as strict as the
         // relative one the estimates' few-ulp error needs
-        if (both && !(fabsf(rs - rz) > 4e-6f)) {
+        if (bs && bz && !(fabsf(dr) > 4e-6f)) {
             rs = ns_ / dsv;
             rz = nz_ / dzv;
+            rs0 = rs;
+            rz0 = rz;
+            dr = rs - rz;
         }
-        const bool sgt = rs > rz;
-        float wg = W;
+        const bool sgt = dr > 0.0f;   // rs > rz
+        const int i2 = sgt ? i2a : i2b;
+        // the three terms' fractions (the reference's branch tree, flattened):
+        // r1 = min(rs, rz), r2 = |rs - rz| (both axes cross), r3 = 1 - m with m
+        // the largest fraction of an axis that crosses
+        const float r1 = min_nonneg(rs, rz);
+        const float r2 = fabsf(dr);
+        const float m = max_nonneg(rs0, rz0);
+        float e = 1.0f;
         if (TOF) {
             const float dd = fmaf(pp1, R.tA, R.tB);
-            wg = W * __builtin_amdgcn_exp2f(-(dd * dd));
+            e = __builtin_amdgcn_exp2f(-(dd * dd));
         }
-        // the three candidate terms (flattened branch tree)
-        const float r1 = bs ? (bz ? (sgt ? rz : rs) : rs) : (bz ? rz : 1.0f);
-        const float r2 = sgt ? (rs - rz) : (rz - rs);
-        const float r3 = bs ? (bz ? (sgt ? 1.0f - rs : 1.0f - rz) : 1.0f - rs) : 1.0f - rz;
-        const int c2s = sgt ? cs1 : cs2, c2z = sgt ? cz2 : cz1;
-        // the reference's validity tests; `c >= 0` parts are implied by the
-        // in-tile tests below (tile origins are >= 0) and left to them
-        const bool v1 = ok;
-        const bool v2 = both & (sgt ? (cz2 < nz) : (cs2 < ns));
-        const bool v3 = ok && (bs || bz) && cs2 < ns && cz2 < nz;
-        // the two secondary and two axial cells relative to the tile: term 1 is
-        // (cs1, cz1), term 3 (cs2, cz2), term 2 (cs1, cz2) or (cs2, cz1)
-        const int rs1 = cs1 - t.s0, rs2 = cs2 - t.s0, rz1 = cz1 - t.z0, rz2 = cz2 - t.z0;
-        const bool ts1 = (unsigned)rs1 < (unsigned)TX, ts2 = (unsigned)rs2 < (unsigned)TX;
-        const bool tz1 = (unsigned)rz1 < (unsigned)TZ, tz2 = (unsigned)rz2 < (unsigned)TZ;
-        const int os1 = (int)((unsigned)rs1 << sstep_log2), os2 = (int)((unsigned)rs2 << sstep_log2);
-        const int oz1 = rz1 * (TX * TY * ES), oz2 = rz2 * (TX * TY * ES);
-        const bool in1 = v1 && ts1 && tz1;
-        const bool in2 = v2 && (sgt ? (ts1 && tz2) : (ts2 && tz1));
-        const bool in3 = v3 && ts2 && tz2;
-        const int i1 = in1 ? base + os1 + oz1 : dummy;
-        const int i2 = in2 ? base + (sgt ? os1 + oz2 : os2 + oz1) : dummy;
-        const int i3 = in3 ? base + os2 + oz2 : dummy;
         if (MODE == M_GATHER) {
             const char *L = reinterpret_cast<const char *>(t.lds);
             const float g1 = *reinterpret_cast<const float *>(L + i1);
             const float g2 = *reinterpret_cast<const float *>(L + i2);
             const float g3 = *reinterpret_cast<const float *>(L + i3);
-            t.acc = fmaf(fmaf(g1, r1, fmaf(g2, r2, g3 * r3)), wg, t.acc);
+            // g3 * (1 - m) as g3 - g3 * m
+            const float sum = fmaf(g1, r1, fmaf(g2, r2, fmaf(-g3, m, g3)));
+            acc = TOF ? fmaf(sum, e, acc) : acc + sum;
         } else if (MODE == M_SCATTER) {
             // float LDS atomics (non-finite values only; see k_bp_tiled)
             char *L = reinterpret_cast<char *>(t.lds);
-            const float vw = t.value * wg;
-            if (in1) atomicAdd(reinterpret_cast<float *>(L + i1), vw * r1);
-            if (in2) atomicAdd(reinterpret_cast<float *>(L + i2), vw * r2);
-            if (in3) atomicAdd(reinterpret_cast<float *>(L + i3), vw * r3);
+            const float v = vw * e, r3 = 1.0f - m;
+            if (i1 != dummy) atomicAdd(reinterpret_cast<float *>(L + i1), v * r1);
+            if (i2 != dummy) atomicAdd(reinterpret_cast<float *>(L + i2), v * r2);
+            if (i3 != dummy) atomicAdd(reinterpret_cast<float *>(L + i3), v * r3);
         } else if (MODE == M_SCATTER_FX) {
-            // 64-bit fixed point: integer LDS atomics run at full rate; t.value
-            // already carries the item's scale 2^shift
+            // 64-bit fixed point: integer LDS atomics run at full rate
             char *L = reinterpret_cast<char *>(t.fx);
-            const float vw = t.value * wg;
-            if (in1) atomicAdd(reinterpret_cast<unsigned long long *>(L + i1), to_fixed_scaled(vw * r1));
-            if (in2) atomicAdd(reinterpret_cast<unsigned long long *>(L + i2), to_fixed_scaled(vw * r2));
-            if (in3) atomicAdd(reinterpret_cast<unsigned long long *>(L + i3), to_fixed_scaled(vw * r3));
+            const double v = TOF ? (double)(vw * e) : vwd;
+            const float r3 = 1.0f - m;
+            if (i1 != dummy) atomicAdd(reinterpret_cast<unsigned long long *>(L + i1), fixed_term(v, r1));
+            if (i2 != dummy) atomicAdd(reinterpret_cast<unsigned long long *>(L + i2), fixed_term(v, r2));
+            if (i3 != dummy) atomicAdd(reinterpret_cast<unsigned long long *>(L + i3), fixed_term(v, r3));
         } else {
             const int pg = ip * t.gsp;
+            const int c2s = sgt ? cs1 : cs2, c2z = sgt ? cz2 : cz1;
             const int bb1 = bs ? (bz ? (sgt ? 5 : 8) : 3) : (bz ? 1 : 0);
             const int bb3 = bs ? (bz ? (sgt ? 7 : 10) : 4) : 2;
-            if (in1) {
+            if (i1 != dummy) {
                 t.h += mix64(((uint64_t)(uint32_t)(pg + cs1 * t.gss + cz1 * t.gsz) << 4) | (uint64_t)bb1);
                 t.cnt++;
             }
-            if (in2) {
+            if (i2 != dummy) {
                 t.h += mix64(((uint64_t)(uint32_t)(pg + c2s * t.gss + c2z * t.gsz) << 4) | (uint64_t)(sgt ? 6 : 9));
                 t.cnt++;
             }
-            if (in3) {
+            if (i3 != dummy) {
                 t.h += mix64(((uint64_t)(uint32_t)(pg + cs2 * t.gss + cz2 * t.gsz) << 4) | (uint64_t)bb3);
                 t.cnt++;
             }
+            ++ip;
         }
         pp1 = pp2;
         S0 = S1;
         Z0 = Z1;
-        base += bstep;
     }
+    if (MODE == M_GATHER) t.acc = acc * W;
 }
 
 __device__ __forceinline__ void tile_origin(int tile, const TileCfg &T, int &x0, int &y0, int &z0)
@@ -624,7 +661,6 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         TileWalk t;
         walk_for(t, fl, TVOX, (R.meta & 1u) != 0, x0, y0, z0);
         t.fx = fx;
-        t.shift = shift;
         t.value = fixed ? ldexpf(vv[k], shift) : vv[k];   // scale folded into the value
         if (fixed) march_tile<TOF, DS, DZ, M_SCATTER_FX>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
         else march_tile<TOF, DS, DZ, M_SCATTER>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
@@ -669,7 +705,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_digest_tiled(const SegRec *__r
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
         const bool xdom = (R.meta & 1u) != 0;
         TileWalk t;
-        walk_for(t, nullptr, 0, xdom, x0, y0, z0);
+        walk_for(t, nullptr, TVOX, xdom, x0, y0, z0);   // dummy: never a tile address
         t.gsp = xdom ? 1 : G.nx;
         t.gss = xdom ? G.nx : 1;
         t.gsz = G.nx * G.ny;
