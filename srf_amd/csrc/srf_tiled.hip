@@ -47,7 +47,14 @@ namespace srf {
 #define SRF_TILE_Z 8
 #endif
 constexpr int TX = 32, TY = 32, TZ = SRF_TILE_Z;
-constexpr int TVOX = TX * TY * TZ;      // 8192 voxels: 32 KiB (float) / 64 KiB (fixed point) of LDS
+constexpr int TVOX = TX * TY * TZ;      // 8192 voxels
+// LDS layout of a tile: cell (lx, ly, lz) at word lx + LSY*ly + LSZ*lz.  The
+// padded strides put a cell in bank (lx + ly + lz) mod 32, so the cells one
+// primary step reaches (a plane of fixed lx for x-dominant rays, of fixed ly
+// for y-dominant ones) spread over the banks instead of sharing one.
+constexpr int LSY = TX + 1, LSZ = LSY * TY + 1;
+constexpr int TPAD = LSZ * (TZ - 1) + LSY * (TY - 1) + TX;   // 8454 words: 33 KiB (float) / 66 KiB (fixed point)
+__device__ __forceinline__ int tile_word(int i) { return (i & (TX - 1)) + LSY * ((i / TX) & (TY - 1)) + LSZ * (i / (TX * TY)); }
 constexpr int NCLASS = 4;               // slab-count classes per tile
 constexpr int ITEM = 8192;              // records per work item (at most)
 constexpr int MAX_KEYS = 16384;         // LDS histogram of the binning kernels (64 KiB): up to 4096 tiles
@@ -297,11 +304,13 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
             R.pad[0] = R.pad[1] = 0u;
             recs[pos] = R;
             rec_ev[pos] = (int)e;
-            // sort key: (tile, class), then slab count (longest first: a wavefront's
-            // lanes march similar lengths), then the segment slot (unique)
+            // sort key: (tile, class), then the branch (a wavefront mostly runs
+            // one branch's marcher), then slab count (longest first: its lanes
+            // march similar lengths), then the segment slot (unique)
             const unsigned len = (unsigned)min(b - a, 63);
-            rec_sk[pos] = ((unsigned long long)(tile * NCLASS + cls) << 38) |
-                          ((unsigned long long)(63u - len) << 32) | (unsigned)R.slot;
+            rec_sk[pos] = ((unsigned long long)(tile * NCLASS + cls) << 39) |
+                          ((unsigned long long)R.meta << 38) | ((unsigned long long)(63u - len) << 32) |
+                          (unsigned)R.slot;
         });
     }
 }
@@ -351,13 +360,11 @@ struct TileWalk {
     float *lds;
     unsigned long long *fx;  // fixed-point accumulator tile
     int dummy;
-    int p0, s0, z0;    // tile origin along primary, secondary, axial
-    int sp, ss;        // LDS strides of primary / secondary
+    int x0, y0, z0;    // tile origin
     float acc;         // gather result (times the segment's weight constant)
     float value;       // scatter value (for fixed point: times 2^shift)
     uint64_t h;        // digest
     int cnt;
-    int gsp, gss, gsz; // global strides (digest)
 };
 
 // One segment: slabs [a, b) of record R inside the walk's tile.
@@ -375,19 +382,23 @@ struct TileWalk {
 //   term 2 (cs1, cz2) if rs > rz, else (cs2, cz1): both crossings, its tile
 //          test, and cz1 >= -1 (resp. cs1 >= -1);
 //   term 3 (cs2, cz2): a crossing, its tile test, cs1 >= -1 and cz1 >= -1.
-template <bool TOF, bool DS, bool DZ, int MODE>
+//
+// XD selects the branch (x- or y-dominant ray), so the tile origin, strides and
+// extents are wave-uniform; a work item's records are sorted by branch.
+template <bool TOF, bool DS, bool DZ, int MODE, bool XD>
 __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G, int a, int b, float W,
                                            TileWalk &t)
 {
-    const bool xdom = (R.meta & 1u) != 0;
     const float p1 = R.p1, sec1 = R.sec1, ks = R.ks, kz = R.kz, z1 = R.z1;
-    const float ds = xdom ? G.dy : 1.0f;
-    const float ns2 = xdom ? G.ny2 : G.nx2;
-    const float np2 = xdom ? G.nx2 : G.ny2;
-    const int ns = xdom ? G.ny : G.nx, nz = G.nz;
+    const float ds = XD ? G.dy : 1.0f;
+    const float ns2 = XD ? G.ny2 : G.nx2;
+    const float np2 = XD ? G.nx2 : G.ny2;
+    const int ns = XD ? G.ny : G.nx, nz = G.nz;
+    const int p0 = XD ? t.x0 : t.y0, s0 = XD ? t.y0 : t.x0, z0 = t.z0;
+    constexpr int SP = XD ? 1 : LSY, SS = XD ? LSY : 1;   // LDS strides of primary / secondary
     const bool ks_pos = ks >= 0.0f, kz_pos = kz >= 0.0f;
     // in-image extent of the tile along secondary / axial
-    const unsigned es = (unsigned)min(TX, ns - t.s0), ez = (unsigned)min(TZ, nz - t.z0);
+    const unsigned es = (unsigned)min(TX, ns - s0), ez = (unsigned)min(TZ, nz - z0);
     float pp1 = (float)a - np2;
     float S0, Z0;
     {
@@ -396,13 +407,12 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const float zv = z1 + kz * (pp1 - p1);
         Z0 = (DZ ? zv / G.dz : zv) + G.nz2;
     }
-    // LDS byte address of cell (c_s, c_z) of slab ip: base + (c_s << sstep_log2)
-    // + (c_z << ZSH), the tile origin folded into base
+    // LDS byte address of cell (c_s, c_z) of slab ip: base + c_s * BS + c_z *
+    // BZ, the tile origin folded into base
     constexpr int ES = (MODE == M_SCATTER_FX) ? 8 : 4;
-    constexpr int ZSH = (ES == 8 ? 3 : 2) + 10;                           // TX * TY = 1024 cells per layer
-    const int sstep_log2 = (t.ss == 1 ? 0 : 5) + (ES == 8 ? 3 : 2);       // secondary stride 1 or TX = 32
-    int base = (a - t.p0) * t.sp * ES - (t.s0 << sstep_log2) - (t.z0 << ZSH);
-    const int bstep = t.sp * ES;
+    constexpr unsigned BS = SS * ES, BZ = LSZ * ES;   // byte strides of secondary / axial
+    int base = (a - p0) * SP * ES - s0 * (int)BS - z0 * (int)BZ;
+    constexpr int bstep = SP * ES;
     const int dummy = t.dummy * ES;
     static_assert(TX == 32 && TY == 32, "secondary stride is 1 or 32");
     float acc = 0.0f;
@@ -410,7 +420,10 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
     const double vwd = (double)vw;
     const int base_end = base + (b - a) * bstep;
     int ip = a;   // digest only
-    for (; base < base_end; base += bstep) {
+    // one slab; the loop below runs it twice per trip (the marcher's inline asm
+    // is convergent, which rules out the compiler's own unrolling), so the
+    // values carried from slab to slab alternate registers instead of moving
+    auto slab = [&]() {
         const float pp2 = pp1 + 1.0f;
         float S1, Z1;
         {
@@ -426,15 +439,18 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const bool bs = cs1 != cs2, bz = cz1 != cz2;
         // LDS addresses of the three terms (dummy where the term is not
         // applied); term 2 is i2a if rs > rz, else i2b
-        const bool ts1 = (unsigned)(cs1 - t.s0) < es, ts2 = (unsigned)(cs2 - t.s0) < es;
-        const bool tz1 = (unsigned)(cz1 - t.z0) < ez, tz2 = (unsigned)(cz2 - t.z0) < ez;
+        const bool ts1 = (unsigned)(cs1 - s0) < es, ts2 = (unsigned)(cs2 - s0) < es;
+        const bool tz1 = (unsigned)(cz1 - z0) < ez, tz2 = (unsigned)(cz2 - z0) < ez;
         const bool los = cs1 >= -1, loz = cz1 >= -1;
-        const int A = base + (cs1 << sstep_log2), B = base + (cs2 << sstep_log2);
-        // (logical operators keep the predicates as lane masks)
-        const int i1 = (ts1 && tz1) ? A + (cz1 << ZSH) : dummy;
-        const int i3 = ((bs || bz) && ts2 && tz2 && los && loz) ? B + (cz2 << ZSH) : dummy;
-        const int i2a = (bs && bz && ts1 && tz2 && loz) ? A + (cz2 << ZSH) : dummy;
-        const int i2b = (bs && bz && ts2 && tz1 && los) ? B + (cz1 << ZSH) : dummy;
+        // (24-bit multiply-adds: a cell whose address is used lies in the tile,
+        // so its coordinates are below 2^24; logical operators keep the
+        // predicates as lane masks)
+        const int A = (int)(__umul24((unsigned)cs1, BS) + (unsigned)base);
+        const int B = (int)(__umul24((unsigned)cs2, BS) + (unsigned)base);
+        const int i1 = (ts1 && tz1) ? (int)(__umul24((unsigned)cz1, BZ) + (unsigned)A) : dummy;
+        const int i3 = ((bs || bz) && ts2 && tz2 && los && loz) ? (int)(__umul24((unsigned)cz2, BZ) + (unsigned)B) : dummy;
+        const int i2a = (bs && bz && ts1 && tz2 && loz) ? (int)(__umul24((unsigned)cz2, BZ) + (unsigned)A) : dummy;
+        const int i2b = (bs && bz && ts2 && tz1 && los) ? (int)(__umul24((unsigned)cz1, BZ) + (unsigned)B) : dummy;
         // materialise the addresses here: sunk past the branch below, their
         // lane-mask operands would be carried across it as 0/1 vectors
         asm volatile("" ::"v"(i1), "v"(i2a), "v"(i2b), "v"(i3));
@@ -456,6 +472,7 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
             rz0 = rz;
             dr = rs - rz;
         }
+        asm("" : "+v"(dr));   // keeps |dr| a source modifier of its uses
         const bool sgt = dr > 0.0f;   // rs > rz
         const int i2 = sgt ? i2a : i2b;
         // the three terms' fractions (the reference's branch tree, flattened):
@@ -493,20 +510,21 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
             if (i2 != dummy) atomicAdd(reinterpret_cast<unsigned long long *>(L + i2), fixed_term(v, r2));
             if (i3 != dummy) atomicAdd(reinterpret_cast<unsigned long long *>(L + i3), fixed_term(v, r3));
         } else {
-            const int pg = ip * t.gsp;
+            const int gsp = XD ? 1 : G.nx, gss = XD ? G.nx : 1, gsz = G.nx * G.ny;
+            const int pg = ip * gsp;
             const int c2s = sgt ? cs1 : cs2, c2z = sgt ? cz2 : cz1;
             const int bb1 = bs ? (bz ? (sgt ? 5 : 8) : 3) : (bz ? 1 : 0);
             const int bb3 = bs ? (bz ? (sgt ? 7 : 10) : 4) : 2;
             if (i1 != dummy) {
-                t.h += mix64(((uint64_t)(uint32_t)(pg + cs1 * t.gss + cz1 * t.gsz) << 4) | (uint64_t)bb1);
+                t.h += mix64(((uint64_t)(uint32_t)(pg + cs1 * gss + cz1 * gsz) << 4) | (uint64_t)bb1);
                 t.cnt++;
             }
             if (i2 != dummy) {
-                t.h += mix64(((uint64_t)(uint32_t)(pg + c2s * t.gss + c2z * t.gsz) << 4) | (uint64_t)(sgt ? 6 : 9));
+                t.h += mix64(((uint64_t)(uint32_t)(pg + c2s * gss + c2z * gsz) << 4) | (uint64_t)(sgt ? 6 : 9));
                 t.cnt++;
             }
             if (i3 != dummy) {
-                t.h += mix64(((uint64_t)(uint32_t)(pg + cs2 * t.gss + cz2 * t.gsz) << 4) | (uint64_t)bb3);
+                t.h += mix64(((uint64_t)(uint32_t)(pg + cs2 * gss + cz2 * gsz) << 4) | (uint64_t)bb3);
                 t.cnt++;
             }
             ++ip;
@@ -514,6 +532,12 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         pp1 = pp2;
         S0 = S1;
         Z0 = Z1;
+        base += bstep;
+    };
+    while (base < base_end) {
+        slab();
+        if (!(base < base_end)) break;
+        slab();
     }
     if (MODE == M_GATHER) t.acc = acc * W;
 }
@@ -525,21 +549,36 @@ __device__ __forceinline__ void tile_origin(int tile, const TileCfg &T, int &x0,
     z0 = (tile / (T.ntx * T.nty)) * TZ;
 }
 
-__device__ __forceinline__ void walk_for(TileWalk &t, float *lds, int dummy, bool xdom, int x0, int y0, int z0)
+// Both branches' marchers; a record's branch bit picks one (the records of a
+// work item are sorted by branch, so a wavefront mostly runs one of them).
+template <bool TOF, bool DS, bool DZ, int MODE>
+__device__ __forceinline__ void march_any(const SegRec &R, const LaunchGeom &G, int a, int b, float W, TileWalk &t)
+{
+    if (R.meta & 1u) march_tile<TOF, DS, DZ, MODE, true>(R, G, a, b, W, t);
+    else march_tile<TOF, DS, DZ, MODE, false>(R, G, a, b, W, t);
+}
+
+__device__ __forceinline__ void walk_for(TileWalk &t, float *lds, int dummy, int x0, int y0, int z0)
 {
     t.lds = lds;
     t.dummy = dummy;
+    t.x0 = x0;
+    t.y0 = y0;
     t.z0 = z0;
-    if (xdom) { t.p0 = x0; t.s0 = y0; t.sp = 1; t.ss = TX; }
-    else { t.p0 = y0; t.s0 = x0; t.sp = TX; t.ss = 1; }
     t.acc = 0.0f;
     t.value = 0.0f;
     t.h = 0;
     t.cnt = 0;
 }
 
-// LDS: the tile plus one zero word (gathers) / one scratch word per lane (scatters).
-constexpr int LDS_WORDS = TVOX + TILE_THREADS;
+// Occupancy.  A 1024-thread workgroup is 4 waves per SIMD and a SIMD holds at
+// most 8 waves: 2 workgroups per CU, which the backprojector's 64 KiB
+// fixed-point tile also allows.  That takes at most 64 VGPRs per wave (the
+// batched backprojector would otherwise use more).
+#define BP_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8)))
+
+// Projector LDS: the tile plus one zero word, the target of rejected terms.
+constexpr int LDS_WORDS = TPAD + 1;
 
 // Projection of one work item: stage its tile in LDS, march every record,
 // one partial sum per record.
@@ -552,15 +591,15 @@ __device__ __forceinline__ void fp_item(const SegRec *__restrict__ recs, const I
     tile_origin(it.tile, T, x0, y0, z0);
     for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) {
         const int gx = x0 + (i % TX), gy = y0 + (i / TX) % TY, gz = z0 + i / (TX * TY);
-        tile[i] = (gx < G.nx && gy < G.ny && gz < G.nz) ? img[gx + G.nx * (gy + G.ny * gz)] : 0.0f;
+        tile[tile_word(i)] = (gx < G.nx && gy < G.ny && gz < G.nz) ? img[gx + G.nx * (gy + G.ny * gz)] : 0.0f;
     }
-    if (threadIdx.x == 0) tile[TVOX] = 0.0f;
+    if (threadIdx.x == 0) tile[TPAD] = 0.0f;
     __syncthreads();
     for (int r = it.start + threadIdx.x; r < it.start + it.count; r += TILE_THREADS) {
         const SegRec R = recs[r];
         TileWalk t;
-        walk_for(t, tile, TVOX, (R.meta & 1u) != 0, x0, y0, z0);
-        march_tile<TOF, DS, DZ, M_GATHER>(R, G, (int)(R.ab_fp & 0xffffu), (int)(R.ab_fp >> 16), R.Wfp, t);
+        walk_for(t, tile, TPAD, x0, y0, z0);
+        march_any<TOF, DS, DZ, M_GATHER>(R, G, (int)(R.ab_fp & 0xffffu), (int)(R.ab_fp >> 16), R.Wfp, t);
         partial[R.slot] = t.acc;
     }
 }
@@ -621,7 +660,7 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         *s_cmax = 0u;
         *s_bad = 0;
     }
-    for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) fx[i] = 0ull;
+    for (int i = threadIdx.x; i < TPAD; i += TILE_THREADS) fx[i] = 0ull;
     const int end = it.start + it.count;
     float vv[RECS_PER_THREAD];
     float vmax = 0.0f;
@@ -650,31 +689,37 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         (void)frexpf(__uint_as_float(*s_cmax), &ex);   // cmax < 2^ex
         shift = FX_BITS - ex;
     }
-#pragma unroll
+    // one marcher body per mode and branch: the values rotate through vv so
+    // the record loop is not unrolled and vv stays in registers
+#pragma unroll 1
     for (int k = 0; k < RECS_PER_THREAD; ++k) {
+        const float v = vv[0];
+#pragma unroll
+        for (int j = 0; j + 1 < RECS_PER_THREAD; ++j) vv[j] = vv[j + 1];
         const int r = it.start + threadIdx.x + k * TILE_THREADS;
-        if (r >= end) continue;
+        if (r >= end) break;
         const SegRec R = recs[r];
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
         const int a = (int)(ab & 0xffffu), b = (int)(ab >> 16);
         if (a >= b) continue;
         TileWalk t;
-        walk_for(t, fl, TVOX, (R.meta & 1u) != 0, x0, y0, z0);
+        walk_for(t, fl, TPAD, x0, y0, z0);
         t.fx = fx;
-        t.value = fixed ? ldexpf(vv[k], shift) : vv[k];   // scale folded into the value
-        if (fixed) march_tile<TOF, DS, DZ, M_SCATTER_FX>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
-        else march_tile<TOF, DS, DZ, M_SCATTER>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
+        t.value = fixed ? ldexpf(v, shift) : v;   // scale folded into the value
+        if (fixed) march_any<TOF, DS, DZ, M_SCATTER_FX>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
+        else march_any<TOF, DS, DZ, M_SCATTER>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
     }
     __syncthreads();
     if (fixed) {
-        for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = ldexpf((float)(long long)fx[i], -shift);
+        for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS)
+            out[i] = ldexpf((float)(long long)fx[tile_word(i)], -shift);
     } else {
-        for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = fl[i];
+        for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = fl[tile_word(i)];
     }
 }
 
 template <bool TOF, bool DS, bool DZ>
-__global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restrict__ recs,
+__global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_tiled(const SegRec *__restrict__ recs,
                                                            const Item *__restrict__ items,
                                                            const float *__restrict__ values,
                                                            const float *__restrict__ y, const float *__restrict__ p,
@@ -682,7 +727,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_bp_tiled(const SegRec *__restr
                                                            const int *__restrict__ rec_ev,
                                                            float *__restrict__ tile_out)
 {
-    __shared__ unsigned long long fx[TVOX];
+    __shared__ unsigned long long fx[TPAD];
     __shared__ unsigned s_cmax;
     __shared__ int s_bad;
     const Item it = items[blockIdx.x];
@@ -703,13 +748,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_digest_tiled(const SegRec *__r
     for (int r = it.start + threadIdx.x; r < it.start + it.count; r += TILE_THREADS) {
         const SegRec R = recs[r];
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
-        const bool xdom = (R.meta & 1u) != 0;
         TileWalk t;
-        walk_for(t, nullptr, TVOX, xdom, x0, y0, z0);   // dummy: never a tile address
-        t.gsp = xdom ? 1 : G.nx;
-        t.gss = xdom ? G.nx : 1;
-        t.gsz = G.nx * G.ny;
-        march_tile<TOF, DS, DZ, M_DIGEST>(R, G, (int)(ab & 0xffffu), (int)(ab >> 16), 0.0f, t);
+        walk_for(t, nullptr, TPAD, x0, y0, z0);   // dummy: never a tile address
+        march_any<TOF, DS, DZ, M_DIGEST>(R, G, (int)(ab & 0xffffu), (int)(ab >> 16), 0.0f, t);
         if (t.cnt) {
             atomicAdd(hash + R.ev, (unsigned long long)t.h);
             atomicAdd(count + R.ev, t.cnt);
@@ -865,11 +906,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fp_batch(const ScanDesc *__res
 }
 
 template <bool TOF, bool DS, bool DZ>
-__global__ __launch_bounds__(TILE_THREADS) void k_bp_batch(const ScanDesc *__restrict__ desc,
+__global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_batch(const ScanDesc *__restrict__ desc,
                                                            const Work *__restrict__ work, int nwork,
                                                            unsigned *__restrict__ counter, int persistent)
 {
-    __shared__ unsigned long long fx[TVOX];
+    __shared__ unsigned long long fx[TPAD];
     __shared__ unsigned s_cmax;
     __shared__ int s_bad;
     __shared__ int s_next;
@@ -1173,10 +1214,10 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
             while ((1 << key_bits) < T.nkeys) ++key_bits;
             size_t sort_bytes = 0;
             if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, sk, sk2, idx, idx2, (int)P->nseg, 0,
-                                                   38 + key_bits, s) != hipSuccess ||
+                                                   39 + key_bits, s) != hipSuccess ||
                 hipMalloc(&sort_tmp, sort_bytes ? sort_bytes : 1) != hipSuccess ||
                 hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, sk, sk2, idx, idx2, (int)P->nseg, 0,
-                                                   38 + key_bits, s) != hipSuccess) {
+                                                   39 + key_bits, s) != hipSuccess) {
                 (void)hipGetLastError();
                 return fail(SRF_ERR_LAUNCH);
             }
