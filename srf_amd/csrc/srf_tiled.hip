@@ -4,7 +4,7 @@
 // Why: with one event per lane every projector term is a gather that misses
 // the per-XCD L2 (a 256x256x128 image is 32 MiB, an XCD's L2 4 MiB) and every
 // backprojector term is a device-scope atomic.  Here the image is cut into
-// TX x TY x TZ = 32 x 32 x 16 voxel tiles (64 KiB) and each event's slab range is
+// TX x TY x TZ = 32 x 32 x 8 voxel tiles and each event's slab range is
 // cut into segments, one per tile it can touch.  A work item is a run of
 // segments of one tile: the projector stages the tile in LDS once and every
 // term is an LDS read; the backprojector accumulates into an LDS tile (ds_add)
@@ -14,13 +14,14 @@
 //   k_bin_count  segments per event and per (tile, length class) key
 //   scan         event offsets (hipcub) and key offsets (host, <= 16384 keys)
 //   k_bin_emit   64-byte segment records, grouped by key: records of one tile
-//                are contiguous and sorted into slab-count classes so the lanes
-//                of a wavefront march similar lengths.
+//                are contiguous and sorted into slab-count classes, then (radix
+//                sort) by branch and slab count, so a wavefront runs one
+//                branch's marcher over similar lengths.
 // A record carries everything the marcher needs (the event's exact branch
 // parameters), so the passes stream records instead of gathering event data.
 //
 // Exactness: segments are a superset (value-space margin 1e-2 voxel, far above
-// float round-off) of the slabs whose terms fall in the tile; inside, march_slabs runs the
+// float round-off) of the slabs whose terms fall in the tile; inside, march_tile runs the
 // reference's exact per-slab arithmetic and the sink keeps only the terms whose
 // cell lies in the tile, so every term of every event is applied exactly once.
 #include <hip/hip_runtime.h>
@@ -351,11 +352,10 @@ __device__ __forceinline__ float max_nonneg(float a, float b)
     return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
 }
 
-// State of one segment walk inside one LDS tile (stored x fastest:
-// lds[lx + TX*(ly + TY*lz)]).  Terms whose cell lies outside the tile, or that
-// the reference's predicates reject, are sent to `dummy`: a zero word for
-// gathers (scatters skip them), so the gather body has no branches and its
-// three LDS reads issue back to back.
+// State of one segment walk inside one LDS tile (layout: tile_word).  Terms
+// whose cell lies outside the tile, or that the reference's predicates reject,
+// are sent to a dummy address (see march_tile), so the slab body has no
+// branches and its three LDS accesses issue back to back.
 struct TileWalk {
     float *lds;
     unsigned long long *fx;  // fixed-point accumulator tile
@@ -413,7 +413,10 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
     constexpr unsigned BS = SS * ES, BZ = LSZ * ES;   // byte strides of secondary / axial
     int base = (a - p0) * SP * ES - s0 * (int)BS - z0 * (int)BZ;
     constexpr int bstep = SP * ES;
-    const int dummy = t.dummy * ES;
+    // rejected terms: a zero word (gathers, never written), the lane's own
+    // slot after the fixed-point tile (fixed-point scatters), or an address
+    // no tile cell has (float scatters and the digest test for it)
+    const int dummy = (MODE == M_SCATTER_FX) ? (t.dummy + (int)threadIdx.x) * ES : t.dummy * ES;
     static_assert(TX == 32 && TY == 32, "secondary stride is 1 or 32");
     float acc = 0.0f;
     const float vw = t.value * W;
@@ -502,13 +505,15 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
             if (i2 != dummy) atomicAdd(reinterpret_cast<float *>(L + i2), v * r2);
             if (i3 != dummy) atomicAdd(reinterpret_cast<float *>(L + i3), v * r3);
         } else if (MODE == M_SCATTER_FX) {
-            // 64-bit fixed point: integer LDS atomics run at full rate
+            // 64-bit fixed point: integer LDS atomics run at full rate.  Every
+            // lane adds all three terms; rejected ones go to the lane's own
+            // dummy slot (no per-term compare and exec-mask save / restore)
             char *L = reinterpret_cast<char *>(t.fx);
             const double v = TOF ? (double)(vw * e) : vwd;
             const float r3 = 1.0f - m;
-            if (i1 != dummy) atomicAdd(reinterpret_cast<unsigned long long *>(L + i1), fixed_term(v, r1));
-            if (i2 != dummy) atomicAdd(reinterpret_cast<unsigned long long *>(L + i2), fixed_term(v, r2));
-            if (i3 != dummy) atomicAdd(reinterpret_cast<unsigned long long *>(L + i3), fixed_term(v, r3));
+            atomicAdd(reinterpret_cast<unsigned long long *>(L + i1), fixed_term(v, r1));
+            atomicAdd(reinterpret_cast<unsigned long long *>(L + i2), fixed_term(v, r2));
+            atomicAdd(reinterpret_cast<unsigned long long *>(L + i3), fixed_term(v, r3));
         } else {
             const int gsp = XD ? 1 : G.nx, gss = XD ? G.nx : 1, gsz = G.nx * G.ny;
             const int pg = ip * gsp;
@@ -595,8 +600,14 @@ __device__ __forceinline__ void fp_item(const SegRec *__restrict__ recs, const I
     }
     if (threadIdx.x == 0) tile[TPAD] = 0.0f;
     __syncthreads();
-    for (int r = it.start + threadIdx.x; r < it.start + it.count; r += TILE_THREADS) {
-        const SegRec R = recs[r];
+    // the next record is loaded while this one is marched
+    const int end = it.start + it.count;
+    int r = it.start + threadIdx.x;
+    SegRec Rn;
+    if (r < end) Rn = recs[r];
+    for (; r < end; r += TILE_THREADS) {
+        const SegRec R = Rn;
+        if (r + TILE_THREADS < end) Rn = recs[r + TILE_THREADS];
         TileWalk t;
         walk_for(t, tile, TPAD, x0, y0, z0);
         march_any<TOF, DS, DZ, M_GATHER>(R, G, (int)(R.ab_fp & 0xffffu), (int)(R.ab_fp >> 16), R.Wfp, t);
@@ -689,16 +700,16 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         (void)frexpf(__uint_as_float(*s_cmax), &ex);   // cmax < 2^ex
         shift = FX_BITS - ex;
     }
-    // one marcher body per mode and branch: the values rotate through vv so
-    // the record loop is not unrolled and vv stays in registers
-#pragma unroll 1
-    for (int k = 0; k < RECS_PER_THREAD; ++k) {
-        const float v = vv[0];
+    // the next record is loaded while this one is marched
+    SegRec Rn;
+    if (it.start + (int)threadIdx.x < end) Rn = recs[it.start + threadIdx.x];
 #pragma unroll
-        for (int j = 0; j + 1 < RECS_PER_THREAD; ++j) vv[j] = vv[j + 1];
+    for (int k = 0; k < RECS_PER_THREAD; ++k) {
+        const float v = vv[k];
         const int r = it.start + threadIdx.x + k * TILE_THREADS;
-        if (r >= end) break;
-        const SegRec R = recs[r];
+        if (r >= end) continue;
+        const SegRec R = Rn;
+        if (r + TILE_THREADS < end) Rn = recs[r + TILE_THREADS];
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
         const int a = (int)(ab & 0xffffu), b = (int)(ab >> 16);
         if (a >= b) continue;
@@ -727,7 +738,7 @@ __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_tiled(const Se
                                                            const int *__restrict__ rec_ev,
                                                            float *__restrict__ tile_out)
 {
-    __shared__ unsigned long long fx[TPAD];
+    __shared__ unsigned long long fx[TPAD + TILE_THREADS];   // tile + one dummy slot per lane
     __shared__ unsigned s_cmax;
     __shared__ int s_bad;
     const Item it = items[blockIdx.x];
@@ -910,7 +921,7 @@ __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_batch(const Sc
                                                            const Work *__restrict__ work, int nwork,
                                                            unsigned *__restrict__ counter, int persistent)
 {
-    __shared__ unsigned long long fx[TPAD];
+    __shared__ unsigned long long fx[TPAD + TILE_THREADS];   // tile + one dummy slot per lane
     __shared__ unsigned s_cmax;
     __shared__ int s_bad;
     __shared__ int s_next;
