@@ -618,11 +618,12 @@ __device__ __forceinline__ void fp_item(const SegRec *__restrict__ recs, const I
 template <bool TOF, bool DS, bool DZ>
 __global__ __launch_bounds__(TILE_THREADS) void k_fp_tiled(const SegRec *__restrict__ recs,
                                                            const Item *__restrict__ items,
+                                                           const int *__restrict__ order,
                                                            const float *__restrict__ img, LaunchGeom G, TileCfg T,
                                                            float *__restrict__ partial)
 {
     __shared__ float tile[LDS_WORDS];
-    const Item it = items[blockIdx.x];
+    const Item it = items[order[blockIdx.x]];
     fp_item<TOF, DS, DZ>(recs, it, img, G, T, partial, tile);
 }
 
@@ -732,6 +733,7 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
 template <bool TOF, bool DS, bool DZ>
 __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_tiled(const SegRec *__restrict__ recs,
                                                            const Item *__restrict__ items,
+                                                           const int *__restrict__ order,
                                                            const float *__restrict__ values,
                                                            const float *__restrict__ y, const float *__restrict__ p,
                                                            int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
@@ -741,9 +743,10 @@ __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_tiled(const Se
     __shared__ unsigned long long fx[TPAD + TILE_THREADS];   // tile + one dummy slot per lane
     __shared__ unsigned s_cmax;
     __shared__ int s_bad;
-    const Item it = items[blockIdx.x];
-    bp_item<TOF, DS, DZ>(recs, it, values, y, p, kind, eps, adjoint, G, T, rec_ev,
-                         tile_out + (int64_t)blockIdx.x * TVOX, fx, &s_cmax, &s_bad);
+    const int k = order[blockIdx.x];
+    const Item it = items[k];
+    bp_item<TOF, DS, DZ>(recs, it, values, y, p, kind, eps, adjoint, G, T, rec_ev, tile_out + (int64_t)k * TVOX, fx,
+                         &s_cmax, &s_bad);
 }
 
 template <bool TOF, bool DS, bool DZ>
@@ -1007,6 +1010,7 @@ struct srf_plan {
     int *rec_ev;
     Item *items;
     int *tile_items;
+    int *order;          // launch order of the items: estimated work, largest first
     int *ev_off;
     uint8_t *ev_flag;
     float *partial;
@@ -1037,6 +1041,7 @@ void plan_free(srf_plan *P)
     (void)hipFree(P->rec_ev);
     (void)hipFree(P->items);
     (void)hipFree(P->tile_items);
+    (void)hipFree(P->order);
     (void)hipFree(P->ev_off);
     (void)hipFree(P->ev_flag);
     (void)hipFree(P->partial);
@@ -1151,15 +1156,18 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
     // key offsets and work items (keys are tile-major, so a tile's items are contiguous)
     std::vector<unsigned> koff((size_t)T.nkeys);
     std::vector<Item> items;
+    std::vector<int> item_cls, order;
     std::vector<int> tile_items((size_t)T.ntiles + 1, 0);
     unsigned acc = 0;
     for (int k = 0; k < T.nkeys; ++k) {
         koff[k] = acc;
         const int tile = k / NCLASS;
         if (k % NCLASS == 0) tile_items[tile] = (int)items.size();
-        for (unsigned st = 0; st < kc[k]; st += item_max)
+        for (unsigned st = 0; st < kc[k]; st += item_max) {
             items.push_back(Item{(int)(acc + st), (int)std::min<unsigned>(item_max, kc[k] - st), tile, 0, 0.0f, 0.0f,
                                  {0.0f, 0.0f}});
+            item_cls.push_back(k % NCLASS);
+        }
         acc += kc[k];
     }
     tile_items[T.ntiles] = (int)items.size();
@@ -1173,11 +1181,22 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
     SRF_TRY(dmalloc(&P->partial, (size_t)P->nseg, P->bytes));
     SRF_TRY(dmalloc(&P->items, items.size(), P->bytes));
     SRF_TRY(dmalloc(&P->tile_items, tile_items.size(), P->bytes));
+    SRF_TRY(dmalloc(&P->order, items.size(), P->bytes));
     SRF_TRY(dmalloc(&P->tile_buf, (size_t)std::max(1, P->nitems) * TVOX, P->bytes));
     if (!items.empty())
         SRF_HIP(hipMemcpyAsync(P->items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
     SRF_HIP(hipMemcpyAsync(P->tile_items, tile_items.data(), tile_items.size() * sizeof(int), hipMemcpyHostToDevice,
                            s));
+    // launch order: estimated work (records times their slab-count class + 1),
+    // largest first, so the tail of a launch is made of small items; the
+    // results do not depend on it (every item writes its own output)
+    order.resize(items.size());
+    for (size_t k = 0; k < items.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return items[(size_t)a].count * (item_cls[(size_t)a] + 1) > items[(size_t)b].count * (item_cls[(size_t)b] + 1);
+    });
+    if (!order.empty())
+        SRF_HIP(hipMemcpyAsync(P->order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s));
     SRF_HIP(hipMemcpyAsync(key_cursor, koff.data(), koff.size() * sizeof(unsigned), hipMemcpyHostToDevice, s));
     if (n > 0) {
         unsigned long long *sk = nullptr, *sk2 = nullptr;
@@ -1291,7 +1310,7 @@ int srf_plan_project(srf_plan *P, const float *image, float *out, void *stream)
     if (!image || !out) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
     if (P->nitems > 0)
-        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, image, P->G, P->T,
+        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, P->order, image, P->G, P->T,
                                                                        P->partial));
     hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial, P->ev_off,
                        P->ev_flag, P->n, (const float *)nullptr, 0, 0.0f, out);
@@ -1307,7 +1326,7 @@ int srf_plan_project_ratio(srf_plan *P, const float *image, const float *y, srf_
     if (!image || !y || !q) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
     if (P->nitems > 0)
-        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, image, P->G, P->T,
+        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, P->order, image, P->G, P->T,
                                                                        P->partial));
     hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial, P->ev_off,
                        P->ev_flag, P->n, y, (int)kind, eps, q);
@@ -1320,7 +1339,7 @@ static int plan_bp(srf_plan *P, const float *values, const float *y, const float
     if (!P || !image) return SRF_ERR_INVALID_ARG;
     if (mode != SRF_BP_REFERENCE && mode != SRF_BP_ADJOINT) return SRF_ERR_INVALID_ARG;
     if (P->nitems > 0)
-        PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, values, y, p, kind, eps,
+        PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, P->order, values, y, p, kind, eps,
                                                                        (int)(mode == SRF_BP_ADJOINT), P->G, P->T,
                                                                        P->rec_ev, P->tile_buf));
     dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
