@@ -229,7 +229,7 @@ typedef struct {
     int32_t persistent;          /* 0: one workgroup per work item; 1: a persistent
                                   * grid that takes items from an atomic queue */
     int32_t workgroups_per_cu;   /* persistent grid = this x compute units (1..16) */
-    int32_t order;               /* 0: scan order; 1: work items by decreasing size */
+    int32_t order;               /* 0: scan order; 1: work items by decreasing estimated work (records x slab-count class) */
 } srf_batch_cfg_t;
 
 int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int32_t n_scans,

@@ -78,7 +78,7 @@ struct __align__(16) SegRec {
 static_assert(sizeof(SegRec) == 64, "record is one 64-byte line");
 
 struct Item {
-    int start, count, tile, pad;
+    int start, count, tile, cls;   // cls: slab-count class of the item's key
     float wmax_bp, wmax_fp;   // largest weight constant of the item's records, per mode
     float pad2[2];
 };
@@ -1000,6 +1000,9 @@ __global__ __launch_bounds__(256) void k_tile_update_batch(const ScanDesc *__res
 
 using namespace srf;
 
+// Estimated work of an item: its records times their slab-count class + 1.
+static inline int64_t item_work(const Item &it) { return (int64_t)it.count * (it.cls + 1); }
+
 struct srf_plan {
     LaunchGeom G;
     TileCfg T;
@@ -1156,7 +1159,7 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
     // key offsets and work items (keys are tile-major, so a tile's items are contiguous)
     std::vector<unsigned> koff((size_t)T.nkeys);
     std::vector<Item> items;
-    std::vector<int> item_cls, order;
+    std::vector<int> order;
     std::vector<int> tile_items((size_t)T.ntiles + 1, 0);
     unsigned acc = 0;
     for (int k = 0; k < T.nkeys; ++k) {
@@ -1164,9 +1167,8 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
         const int tile = k / NCLASS;
         if (k % NCLASS == 0) tile_items[tile] = (int)items.size();
         for (unsigned st = 0; st < kc[k]; st += item_max) {
-            items.push_back(Item{(int)(acc + st), (int)std::min<unsigned>(item_max, kc[k] - st), tile, 0, 0.0f, 0.0f,
-                                 {0.0f, 0.0f}});
-            item_cls.push_back(k % NCLASS);
+            items.push_back(Item{(int)(acc + st), (int)std::min<unsigned>(item_max, kc[k] - st), tile, k % NCLASS,
+                                 0.0f, 0.0f, {0.0f, 0.0f}});
         }
         acc += kc[k];
     }
@@ -1193,7 +1195,7 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
     order.resize(items.size());
     for (size_t k = 0; k < items.size(); ++k) order[k] = (int)k;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return items[(size_t)a].count * (item_cls[(size_t)a] + 1) > items[(size_t)b].count * (item_cls[(size_t)b] + 1);
+        return item_work(items[(size_t)a]) > item_work(items[(size_t)b]);
     });
     if (!order.empty())
         SRF_HIP(hipMemcpyAsync(P->order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s));
@@ -1455,7 +1457,8 @@ int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int3
     std::vector<int64_t> ev_base((size_t)n + 1, 0);
     std::vector<int> tile_base((size_t)n + 1, 0);
     std::vector<Work> work;
-    std::vector<int> cost, variant;
+    std::vector<int64_t> cost;
+    std::vector<int> variant;
     for (int i = 0; i < n; ++i) {
         const srf_plan *P = plans[i];
         ScanDesc &d = desc[(size_t)i];
@@ -1482,7 +1485,7 @@ int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int3
             SRF_HIP(hipStreamSynchronize(s));
             for (int k = 0; k < P->nitems; ++k) {
                 work.push_back(Work{i, k});
-                cost.push_back(items[(size_t)k].count);
+                cost.push_back(item_work(items[(size_t)k]));
                 variant.push_back(plan_variant(P));
             }
         }
