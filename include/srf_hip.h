@@ -333,6 +333,15 @@ int srf_cnn_features(const float *em_int, const float *mu_int, const float *extr
 /* f32 <-> bf16 (round to nearest even). */
 int srf_cnn_cast(const void *x, int64_t n, int32_t to_bf16, void *y, void *stream);
 
+/* ---- crystal geometry ---------------------------------------------------
+ * LORs of the ordered crystal pairs p in [pair_start, pair_start + n_pairs):
+ * lors[k] = (pos[p / N], pos[p % N]) (6 floats), N = n_crystals, from the
+ * crystal-centre table crystal_pos (N x 3).  Replaces the host LOR arrays that
+ * srfnef's EmapGenerator builds with np.kron (emap_generator.py:28-119,
+ * 122-243); the pair order is theirs. */
+int srf_pair_lors(const float *crystal_pos, int64_t n_crystals, int64_t pair_start, int64_t n_pairs, float *lors,
+                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,7 @@ SIGNATURES = [
     ("srf_em_update", ctypes.c_int, [_P, _P, _P, _I64, _F, _P]),
     ("srf_lor_digest", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.c_int, _P, _P, _P]),
     ("srf_smooth", ctypes.c_int, [_P, _I32, _I32, _I32, _F, _F, _F, _P, _P, _P, _P]),
+    ("srf_pair_lors", ctypes.c_int, [_P, _I64, _I64, _I64, _P, _P]),
     ("srf_image_to_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
     ("srf_image_from_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
     ("srf_plan_create", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(_P), _P]),

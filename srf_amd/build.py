@@ -20,7 +20,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsrf_hip.so")
 
-SOURCES = ["srf_project.hip", "srf_elementwise.hip", "srf_tiled.hip", "srf_sss.hip", "srf_cnn.hip"]
+SOURCES = ["srf_project.hip", "srf_elementwise.hip", "srf_tiled.hip", "srf_sss.hip", "srf_cnn.hip", "srf_geometry.hip"]
 
 
 def hipcc():

@@ -387,17 +387,31 @@ def uniform_cylinder_lors(n, radius, axial_length, seed, device_=None):
 class EmapGenerator:
     """Sensitivity image = backprojection of ones over a LOR set.
 
-    The reference enumerates crystal pairs of a block scanner
-    (emap_generator.py:120-243).  For an ideal cylinder this builds the emap as
-    the backprojection of a seeded sample of uniform endpoint pairs (n_lors,
-    default 4e6), scaled by 1/n_lors -- the synthetic-data decision recorded
-    for the configs without crystals."""
+    For a crystal-level scanner (scanner.PetEcatScanner,
+    scanner.PetCylindricalScanner) it is srfnef's sum over every ordered
+    crystal pair (emap_generator.py:28-119 ECAT, :122-243 cylindrical,
+    :246-257 dispatch): crystal_pair_emap.  Every mode gives that exact sum
+    ('full', 'block-full', 'rsector-full' and 'auto' enumerate it; the
+    reference's 'block' / 'rsector' approximate it by rotating block-pair
+    images, whose interpolating rotation op is not part of this path).
+
+    For an ideal cylinder (CylinderScanner) it is the backprojection of a
+    seeded sample of uniform endpoint pairs (n_lors, default 4e6), scaled by
+    1/n_lors -- the synthetic-data decision recorded for the configs without
+    crystals."""
     mode: str
     scanner: object
     n_lors: int = 4_000_000
     seed: int = 12345
 
     def __call__(self, image_config) -> Emap:
+        from .scanner import PetCylindricalScanner, PetEcatScanner
+        if isinstance(self.scanner, (PetEcatScanner, PetCylindricalScanner)):
+            if self.mode not in ("full", "block", "block-full", "rsector", "rsector-full", "auto"):
+                raise NotImplementedError(self.mode)
+            emap_k = crystal_pair_emap(self.scanner, image_config)
+            return Emap(image_from_kernel(emap_k).cpu().numpy(), list(image_config.center),
+                        list(image_config.size))
         sc = self.scanner
         lors = uniform_cylinder_lors(int(self.n_lors), float(sc.inner_radius), float(sc.axial_length), self.seed,
                                      device())
@@ -409,6 +423,50 @@ class EmapGenerator:
         return Emap(emap.cpu().numpy(), list(image_config.center), list(image_config.size))
 
 
+def crystal_pair_emap(scanner, image_config, chunk_pairs=1 << 23, item_records=8192, stats=None):
+    """Sum over every ordered crystal pair (a, b) of the reference
+    backprojection of 1 along the LOR from crystal a to crystal b, in the GPU
+    kernel layout (nz, ny, nx).
+
+    The crystal-centre table is built once; the pair LORs are generated on the
+    GPU chunk by chunk (srf_pair_lors, chunk_pairs pairs at a time), each chunk
+    is binned into a tiled plan and backprojected into the accumulator
+    (srf_plan_backproject with accumulate).  Self pairs and pairs whose
+    transaxial extent is below 10 voxels contribute nothing, as in the
+    reference backprojector (siddon.cu.cc:364)."""
+    from .scanner import crystal_positions
+    from ._lib import check, lib, stream_handle
+    dev = device()
+    pos = torch.from_numpy(crystal_positions(scanner)).to(dev)
+    ncrys = int(pos.shape[0])
+    total = ncrys * ncrys
+    shape = _shape(image_config)
+    geom = _geom(image_config)
+    nx, ny, nz = shape
+    acc = torch.zeros((nz, ny, nx), dtype=torch.float32, device=dev)
+    chunk = int(min(chunk_pairs, total))
+    lors = torch.empty((chunk, 6), dtype=torch.float32, device=dev)
+    nseg = 0
+    with torch.cuda.device(dev):
+        for p0 in range(0, total, chunk):
+            n = min(chunk, total - p0)
+            check(lib().srf_pair_lors(pos.data_ptr(), ncrys, p0, n, lors.data_ptr(), stream_handle(dev)),
+                  "srf_pair_lors")
+            plan = ops.Plan(lors[:n], shape, geom, False, item_records=item_records)
+            try:
+                check(lib().srf_plan_backproject(plan.handle, None, 0, 1, acc.data_ptr(), stream_handle(dev)),
+                      "srf_plan_backproject")
+                if stats is not None:
+                    nseg += plan.info()["segments"]
+            finally:
+                plan.close()
+    if stats is not None:
+        stats.update({"crystals": ncrys, "pairs": total, "chunks": (total + chunk - 1) // chunk,
+                      "segments": nseg})
+    return acc
+
+
 __all__ = ["Project", "ProjectTof", "BackProject", "BackProjectTof", "Mlem", "MlemFull", "MlemFullTof",
            "OsemFull", "OsemTof", "EmapGenerator", "CylinderScanner", "EmEngine", "OsemEngine",
-           "uniform_cylinder_lors", "subset_permutation", "to_gpu", "project_data", "backproject_data"]
+           "uniform_cylinder_lors", "subset_permutation", "to_gpu", "project_data", "backproject_data",
+           "crystal_pair_emap"]
