@@ -147,10 +147,11 @@ int srf_image_from_kernel_layout(const float *src_kernel, float *dst_xyz, int32_
  * (siddon.cu.cc:706-753, siddon_tof.cu.cc:761-812); results agree with them to
  * float round-off (summation order differs).
  *
- * srf_plan_create and srf_plan_destroy are the only entry points that allocate
- * or free device memory (the binned records, about 64 B per event-tile segment,
- * plus per-item tile buffers); srf_plan_info reports the total.  The LOR array
- * is read during srf_plan_create only.  create synchronises `stream`. */
+ * srf_plan_create, srf_plan_rebuild and srf_plan_destroy are the only entry
+ * points that allocate or free device memory (the binned records, about 64 B
+ * per event-tile segment, plus per-item tile buffers); srf_plan_info reports
+ * the total.  The LOR array is read during create / rebuild only, which
+ * synchronise `stream`. */
 typedef struct srf_plan srf_plan;
 
 int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride,
@@ -176,6 +177,15 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride,
                        const float *tof_values, int64_t tof_stride,
                        srf_grid_t grid, const srf_tof_t *tof, const srf_plan_opts_t *opts,
                        srf_plan **plan, void *stream);
+/* Bins a new event list into an existing plan, reusing its device buffers:
+ * they are grown when the new list needs more and otherwise kept, with the
+ * binning scratch, so a loop over chunks of one size (the sensitivity image
+ * over all crystal pairs) allocates only on its first pass.  Arguments as
+ * srf_plan_create_ex; on failure the plan holds an empty list.  No reference
+ * counterpart (srfnef re-runs its TF op per call). */
+int srf_plan_rebuild(srf_plan *plan, const float *lors, int64_t n, int64_t lor_stride,
+                     const float *tof_values, int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof,
+                     const srf_plan_opts_t *opts, void *stream);
 int srf_plan_destroy(srf_plan *plan);
 int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments,
                   int64_t *n_items, size_t *device_bytes);

@@ -72,6 +72,8 @@ SIGNATURES = [
     ("srf_plan_create", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(_P), _P]),
     ("srf_plan_create_ex", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(SrfPlanOpts),
                                           ctypes.POINTER(_P), _P]),
+    ("srf_plan_rebuild", ctypes.c_int, [_P, _P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(SrfPlanOpts),
+                                         _P]),
     ("srf_plan_destroy", ctypes.c_int, [_P]),
     ("srf_batch_create", ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(SrfBatchScan), _I32,
                                         ctypes.POINTER(SrfBatchCfg), ctypes.POINTER(_P), _P]),

@@ -1019,22 +1019,36 @@ using namespace srf;
 // Estimated work of an item: its records times their slab-count class + 1.
 static inline int64_t item_work(const Item &it) { return (int64_t)it.count * (it.cls + 1); }
 
+// A device buffer with its capacity in elements (plans that are rebuilt keep
+// their buffers and only grow them).
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+};
+
 struct srf_plan {
     LaunchGeom G;
     TileCfg T;
     bool tof, ds, dz;
     int64_t n, nseg;
     int nitems;
-    SegRec *recs;
-    int *rec_ev;
-    Item *items;
-    int *tile_items;
-    int *order;          // launch order of the items: estimated work, largest first
-    int *ev_off;
-    uint8_t *ev_flag;
-    float *partial;
-    float *tile_buf;
-    size_t bytes;
+    DevBuf<SegRec> recs;
+    DevBuf<int> rec_ev;
+    DevBuf<Item> items;
+    DevBuf<int> tile_items;
+    DevBuf<int> order;   // launch order of the items: estimated work, largest first
+    DevBuf<int> ev_off;
+    DevBuf<uint8_t> ev_flag;
+    DevBuf<float> partial;
+    DevBuf<float> tile_buf;
+    // binning scratch: freed after srf_plan_create, kept by srf_plan_rebuild
+    DevBuf<int> ev_cnt;
+    DevBuf<unsigned> key_count, key_cursor;
+    DevBuf<char> scan_tmp, sort_tmp;
+    DevBuf<unsigned long long> sk, sk2;
+    DevBuf<int> idx, idx2, rec_ev2;
+    DevBuf<SegRec> recs2;
 };
 
 namespace {
@@ -1053,23 +1067,86 @@ int dmalloc(T **p, size_t count, size_t &total)
     return SRF_OK;
 }
 
+template <class T>
+void dfree(DevBuf<T> &b)
+{
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// At least `count` elements (1 if count is 0), reallocated only to grow.
+template <class T>
+int ensure(DevBuf<T> &b, size_t count)
+{
+    if (count == 0) count = 1;
+    if (b.cap >= count) return SRF_OK;
+    dfree(b);
+    if (hipMalloc((void **)&b.p, count * sizeof(T)) != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return SRF_ERR_LAUNCH;
+    }
+    b.cap = count;
+    return SRF_OK;
+}
+
+void free_scratch(srf_plan *P)
+{
+    dfree(P->ev_cnt);
+    dfree(P->key_count);
+    dfree(P->key_cursor);
+    dfree(P->scan_tmp);
+    dfree(P->sort_tmp);
+    dfree(P->sk);
+    dfree(P->sk2);
+    dfree(P->idx);
+    dfree(P->idx2);
+    dfree(P->rec_ev2);
+    dfree(P->recs2);
+}
+
 void plan_free(srf_plan *P)
 {
     if (!P) return;
-    (void)hipFree(P->recs);
-    (void)hipFree(P->rec_ev);
-    (void)hipFree(P->items);
-    (void)hipFree(P->tile_items);
-    (void)hipFree(P->order);
-    (void)hipFree(P->ev_off);
-    (void)hipFree(P->ev_flag);
-    (void)hipFree(P->partial);
-    (void)hipFree(P->tile_buf);
+    dfree(P->recs);
+    dfree(P->rec_ev);
+    dfree(P->items);
+    dfree(P->tile_items);
+    dfree(P->order);
+    dfree(P->ev_off);
+    dfree(P->ev_flag);
+    dfree(P->partial);
+    dfree(P->tile_buf);
+    free_scratch(P);
     delete P;
+}
+
+size_t plan_bytes(const srf_plan *P)
+{
+    return P->recs.cap * sizeof(SegRec) + P->rec_ev.cap * sizeof(int) + P->items.cap * sizeof(Item) +
+           P->tile_items.cap * sizeof(int) + P->order.cap * sizeof(int) + P->ev_off.cap * sizeof(int) +
+           P->ev_flag.cap + P->partial.cap * sizeof(float) + P->tile_buf.cap * sizeof(float);
 }
 
 }  // namespace
 
+// plan_build: return the error code (the caller decides what to free)
+#define PB_TRY(x)                 \
+    do {                          \
+        int rc__ = (x);           \
+        if (rc__) return rc__;    \
+    } while (0)
+
+#define PB_HIP(x)                                   \
+    do {                                            \
+        if ((x) != hipSuccess) {                    \
+            (void)hipGetLastError();                \
+            return SRF_ERR_LAUNCH;                  \
+        }                                           \
+    } while (0)
+
+// entry points with a local cleanup(): run it, then return the error code
 #define SRF_TRY(x)                \
     do {                          \
         int rc__ = (x);           \
@@ -1088,20 +1165,14 @@ void plan_free(srf_plan *P)
         }                                           \
     } while (0)
 
-extern "C" {
+namespace {
 
-int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const float *tof_values, int64_t tof_stride,
-                    srf_grid_t grid, const srf_tof_t *tof, srf_plan **out, void *stream)
+// Bins an event list into P (whose buffers are grown as needed and otherwise
+// reused); on failure P holds no valid list.
+int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, const float *tof_values,
+               int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof, const srf_plan_opts_t *opts,
+               hipStream_t s)
 {
-    return srf_plan_create_ex(lors, n, lor_stride, tof_values, tof_stride, grid, tof, nullptr, out, stream);
-}
-
-int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const float *tof_values,
-                       int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof, const srf_plan_opts_t *opts,
-                       srf_plan **out, void *stream)
-{
-    if (!out) return SRF_ERR_INVALID_ARG;
-    *out = nullptr;
     const int32_t item_records = (opts && opts->item_records) ? opts->item_records : ITEM;
     const float twin = opts ? opts->tof_window : 0.0f;
     if (item_records < 256 || item_records > ITEM) return SRF_ERR_INVALID_ARG;
@@ -1121,57 +1192,46 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
     T.ntiles = T.ntx * T.nty * T.ntz;
     T.nkeys = T.ntiles * NCLASS;
     if (T.nkeys > MAX_KEYS) return SRF_ERR_UNSUPPORTED;
-    hipStream_t s = (hipStream_t)stream;
 
-    srf_plan *P = new srf_plan();
-    memset(P, 0, sizeof(*P));
+    P->n = 0;
+    P->nseg = 0;
+    P->nitems = 0;
     P->G = G;
     P->T = T;
     P->tof = tof != nullptr;
     P->ds = G.dy != 1.0f;
     P->dz = G.dz != 1.0f;
-    P->n = n;
-    unsigned *key_count = nullptr, *key_cursor = nullptr;
-    int *ev_cnt = nullptr;
-    void *scan_tmp = nullptr;
-    auto cleanup = [&]() {
-        (void)hipFree(ev_cnt);
-        (void)hipFree(key_count);
-        (void)hipFree(key_cursor);
-        (void)hipFree(scan_tmp);
-        plan_free(P);
-    };
-    size_t scratch = 0;
-    SRF_TRY(dmalloc(&P->ev_off, (size_t)n + 1, P->bytes));
-    SRF_TRY(dmalloc(&P->ev_flag, (size_t)n, P->bytes));
-    SRF_TRY(dmalloc(&ev_cnt, (size_t)n + 1, scratch));
-    SRF_TRY(dmalloc(&key_count, (size_t)T.nkeys, scratch));
-    SRF_TRY(dmalloc(&key_cursor, (size_t)T.nkeys, scratch));
-    hipLaunchKernelGGL(k_fill_u32, dim3((T.nkeys + 255) / 256), dim3(256), 0, s, key_count, 0u, (int64_t)T.nkeys);
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, (unsigned *)ev_cnt, 0u,
-                       n + 1);
+    PB_TRY(ensure(P->ev_off, (size_t)n + 1));
+    PB_TRY(ensure(P->ev_flag, (size_t)n));
+    PB_TRY(ensure(P->ev_cnt, (size_t)n + 1));
+    PB_TRY(ensure(P->key_count, (size_t)T.nkeys));
+    PB_TRY(ensure(P->key_cursor, (size_t)T.nkeys));
+    hipLaunchKernelGGL(k_fill_u32, dim3((T.nkeys + 255) / 256), dim3(256), 0, s, P->key_count.p, 0u,
+                       (int64_t)T.nkeys);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s,
+                       (unsigned *)P->ev_cnt.p, 0u, n + 1);
     const unsigned nb = (unsigned)((n + BIN_THREADS - 1) / BIN_THREADS);
     const size_t shm = (size_t)T.nkeys * sizeof(unsigned);
     if (n > 0) {
         if (P->tof)
             hipLaunchKernelGGL(k_bin_count<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, ev_cnt, P->ev_flag, key_count);
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_cnt.p, P->ev_flag.p, P->key_count.p);
         else
             hipLaunchKernelGGL(k_bin_count<false>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, ev_cnt, P->ev_flag, key_count);
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_cnt.p, P->ev_flag.p, P->key_count.p);
     }
-    SRF_HIP(hipGetLastError());
+    PB_HIP(hipGetLastError());
     // event offsets: exclusive scan of the per-event counts (ev_off[n] = total)
     size_t tmp_bytes = 0;
-    SRF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ev_cnt, P->ev_off, (int)(n + 1), s));
-    SRF_HIP(hipMalloc(&scan_tmp, tmp_bytes ? tmp_bytes : 1));
-    SRF_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp_bytes, ev_cnt, P->ev_off, (int)(n + 1), s));
+    PB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, P->ev_cnt.p, P->ev_off.p, (int)(n + 1), s));
+    PB_TRY(ensure(P->scan_tmp, tmp_bytes));
+    PB_HIP(hipcub::DeviceScan::ExclusiveSum(P->scan_tmp.p, tmp_bytes, P->ev_cnt.p, P->ev_off.p, (int)(n + 1), s));
     std::vector<unsigned> kc((size_t)T.nkeys);
     int total = 0;
-    SRF_HIP(hipMemcpyAsync(kc.data(), key_count, kc.size() * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    SRF_HIP(hipMemcpyAsync(&total, P->ev_off + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    SRF_HIP(hipStreamSynchronize(s));
-    P->nseg = total;
+    PB_HIP(hipMemcpyAsync(kc.data(), P->key_count.p, kc.size() * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    PB_HIP(hipMemcpyAsync(&total, P->ev_off.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    PB_HIP(hipStreamSynchronize(s));
+    const int64_t nseg = total;
     // key offsets and work items (keys are tile-major, so a tile's items are contiguous)
     std::vector<unsigned> koff((size_t)T.nkeys);
     std::vector<Item> items;
@@ -1189,22 +1249,19 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
         acc += kc[k];
     }
     tile_items[T.ntiles] = (int)items.size();
-    if ((int64_t)acc != P->nseg) {
-        cleanup();
-        return SRF_ERR_LAUNCH;
-    }
-    P->nitems = (int)items.size();
-    SRF_TRY(dmalloc(&P->recs, (size_t)P->nseg, P->bytes));
-    SRF_TRY(dmalloc(&P->rec_ev, (size_t)P->nseg, P->bytes));
-    SRF_TRY(dmalloc(&P->partial, (size_t)P->nseg, P->bytes));
-    SRF_TRY(dmalloc(&P->items, items.size(), P->bytes));
-    SRF_TRY(dmalloc(&P->tile_items, tile_items.size(), P->bytes));
-    SRF_TRY(dmalloc(&P->order, items.size(), P->bytes));
-    SRF_TRY(dmalloc(&P->tile_buf, (size_t)std::max(1, P->nitems) * TVOX, P->bytes));
+    if ((int64_t)acc != nseg) return SRF_ERR_LAUNCH;
+    const int nitems = (int)items.size();
+    PB_TRY(ensure(P->recs, (size_t)nseg));
+    PB_TRY(ensure(P->rec_ev, (size_t)nseg));
+    PB_TRY(ensure(P->partial, (size_t)nseg));
+    PB_TRY(ensure(P->items, items.size()));
+    PB_TRY(ensure(P->tile_items, tile_items.size()));
+    PB_TRY(ensure(P->order, items.size()));
+    PB_TRY(ensure(P->tile_buf, (size_t)std::max(1, nitems) * TVOX));
     if (!items.empty())
-        SRF_HIP(hipMemcpyAsync(P->items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
-    SRF_HIP(hipMemcpyAsync(P->tile_items, tile_items.data(), tile_items.size() * sizeof(int), hipMemcpyHostToDevice,
-                           s));
+        PB_HIP(hipMemcpyAsync(P->items.p, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
+    PB_HIP(hipMemcpyAsync(P->tile_items.p, tile_items.data(), tile_items.size() * sizeof(int),
+                           hipMemcpyHostToDevice, s));
     // launch order: estimated work (records times their slab-count class + 1),
     // largest first, so the tail of a launch is made of small items; the
     // results do not depend on it (every item writes its own output)
@@ -1214,83 +1271,100 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
         return item_work(items[(size_t)a]) > item_work(items[(size_t)b]);
     });
     if (!order.empty())
-        SRF_HIP(hipMemcpyAsync(P->order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    SRF_HIP(hipMemcpyAsync(key_cursor, koff.data(), koff.size() * sizeof(unsigned), hipMemcpyHostToDevice, s));
+        PB_HIP(hipMemcpyAsync(P->order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    PB_HIP(hipMemcpyAsync(P->key_cursor.p, koff.data(), koff.size() * sizeof(unsigned), hipMemcpyHostToDevice, s));
     if (n > 0) {
-        unsigned long long *sk = nullptr, *sk2 = nullptr;
-        int *idx = nullptr, *idx2 = nullptr;
-        SegRec *recs2 = nullptr;
-        int *rec_ev2 = nullptr;
-        void *sort_tmp = nullptr;
-        auto cleanup_sort = [&]() {
-            (void)hipFree(sk);
-            (void)hipFree(sk2);
-            (void)hipFree(idx);
-            (void)hipFree(idx2);
-            (void)hipFree(sort_tmp);
-        };
-        auto fail = [&](int rc) {
-            cleanup_sort();
-            (void)hipFree(recs2);
-            (void)hipFree(rec_ev2);
-            cleanup();
-            return rc;
-        };
-        size_t tmp = 0;
-        if (dmalloc(&sk, (size_t)P->nseg, tmp) || dmalloc(&sk2, (size_t)P->nseg, tmp) ||
-            dmalloc(&idx, (size_t)P->nseg, tmp) || dmalloc(&idx2, (size_t)P->nseg, tmp) ||
-            dmalloc(&recs2, (size_t)P->nseg, tmp) || dmalloc(&rec_ev2, (size_t)P->nseg, tmp))
-            return fail(SRF_ERR_LAUNCH);
+        PB_TRY(ensure(P->sk, (size_t)nseg));
+        PB_TRY(ensure(P->sk2, (size_t)nseg));
+        PB_TRY(ensure(P->idx, (size_t)nseg));
+        PB_TRY(ensure(P->idx2, (size_t)nseg));
+        PB_TRY(ensure(P->recs2, (size_t)nseg));
+        PB_TRY(ensure(P->rec_ev2, (size_t)nseg));
         if (P->tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, P->ev_off, key_cursor, P->recs, P->rec_ev,
-                               sk);
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->key_cursor.p, P->recs.p,
+                               P->rec_ev.p, P->sk.p);
         else
             hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, P->ev_off, key_cursor, P->recs, P->rec_ev,
-                               sk);
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->key_cursor.p, P->recs.p,
+                               P->rec_ev.p, P->sk.p);
         // Deterministic record order: the emit places each record inside its key's
         // range through atomic cursors, so the order within a key (and with it the
         // split into work items and each item's fixed-point scale) would vary from
-        // run to run.  Sorting by (key, slab count, segment slot) fixes it -- every
-        // plan of the same events is bit-identical -- and hands each wavefront
-        // segments of similar length.
-        if (P->nseg > 0) {
-            const unsigned nbs = (unsigned)((P->nseg + 255) / 256);
-            hipLaunchKernelGGL(k_iota_i32, dim3(nbs), dim3(256), 0, s, idx, P->nseg);
+        // run to run.  Sorting by (key, branch, slab count, segment slot) fixes it
+        // -- every plan of the same events is bit-identical -- and hands each
+        // wavefront one branch's segments of similar length.
+        if (nseg > 0) {
+            const unsigned nbs = (unsigned)((nseg + 255) / 256);
+            hipLaunchKernelGGL(k_iota_i32, dim3(nbs), dim3(256), 0, s, P->idx.p, nseg);
             int key_bits = 1;
             while ((1 << key_bits) < T.nkeys) ++key_bits;
             size_t sort_bytes = 0;
-            if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, sk, sk2, idx, idx2, (int)P->nseg, 0,
-                                                   39 + key_bits, s) != hipSuccess ||
-                hipMalloc(&sort_tmp, sort_bytes ? sort_bytes : 1) != hipSuccess ||
-                hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, sk, sk2, idx, idx2, (int)P->nseg, 0,
-                                                   39 + key_bits, s) != hipSuccess) {
-                (void)hipGetLastError();
-                return fail(SRF_ERR_LAUNCH);
-            }
-            hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs, P->rec_ev, idx2, P->nseg, recs2,
-                               rec_ev2);
-            if (hipStreamSynchronize(s) != hipSuccess) {
-                (void)hipGetLastError();
-                return fail(SRF_ERR_LAUNCH);
-            }
-            std::swap(P->recs, recs2);
-            std::swap(P->rec_ev, rec_ev2);
+            PB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, P->sk.p, P->sk2.p, P->idx.p, P->idx2.p,
+                                                       (int)nseg, 0, 39 + key_bits, s));
+            PB_TRY(ensure(P->sort_tmp, sort_bytes));
+            PB_HIP(hipcub::DeviceRadixSort::SortPairs(P->sort_tmp.p, sort_bytes, P->sk.p, P->sk2.p, P->idx.p,
+                                                       P->idx2.p, (int)nseg, 0, 39 + key_bits, s));
+            hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs.p, P->rec_ev.p, P->idx2.p, nseg,
+                               P->recs2.p, P->rec_ev2.p);
+            PB_HIP(hipGetLastError());
+            std::swap(P->recs, P->recs2);
+            std::swap(P->rec_ev, P->rec_ev2);
         }
-        cleanup_sort();
-        (void)hipFree(recs2);
-        (void)hipFree(rec_ev2);
     }
-    if (P->nitems > 0) hipLaunchKernelGGL(k_item_wmax, dim3((unsigned)P->nitems), dim3(256), 0, s, P->recs, P->items);
-    SRF_HIP(hipGetLastError());
-    SRF_HIP(hipStreamSynchronize(s));
-    (void)hipFree(ev_cnt);
-    (void)hipFree(key_count);
-    (void)hipFree(key_cursor);
-    (void)hipFree(scan_tmp);
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_item_wmax, dim3((unsigned)nitems), dim3(256), 0, s, P->recs.p, P->items.p);
+    PB_HIP(hipGetLastError());
+    PB_HIP(hipStreamSynchronize(s));
+    P->n = n;
+    P->nseg = nseg;
+    P->nitems = nitems;
+    return SRF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride, const float *tof_values, int64_t tof_stride,
+                    srf_grid_t grid, const srf_tof_t *tof, srf_plan **out, void *stream)
+{
+    return srf_plan_create_ex(lors, n, lor_stride, tof_values, tof_stride, grid, tof, nullptr, out, stream);
+}
+
+int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const float *tof_values,
+                       int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof, const srf_plan_opts_t *opts,
+                       srf_plan **out, void *stream)
+{
+    if (!out) return SRF_ERR_INVALID_ARG;
+    *out = nullptr;
+    srf_plan *P = new srf_plan();
+    const int rc = plan_build(P, lors, n, lor_stride, tof_values, tof_stride, grid, tof, opts, (hipStream_t)stream);
+    if (rc) {
+        (void)hipStreamSynchronize((hipStream_t)stream);
+        (void)hipGetLastError();
+        plan_free(P);
+        return rc;
+    }
+    free_scratch(P);
     *out = P;
     return SRF_OK;
+}
+
+int srf_plan_rebuild(srf_plan *plan, const float *lors, int64_t n, int64_t lor_stride, const float *tof_values,
+                     int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof, const srf_plan_opts_t *opts,
+                     void *stream)
+{
+    if (!plan) return SRF_ERR_INVALID_ARG;
+    const int rc = plan_build(plan, lors, n, lor_stride, tof_values, tof_stride, grid, tof, opts,
+                              (hipStream_t)stream);
+    if (rc) {
+        (void)hipStreamSynchronize((hipStream_t)stream);
+        (void)hipGetLastError();
+        plan->n = plan->nseg = 0;
+        plan->nitems = 0;
+    }
+    return rc;
 }
 
 int srf_plan_destroy(srf_plan *plan)
@@ -1306,7 +1380,7 @@ int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments, 
     if (n_events) *n_events = plan->n;
     if (n_segments) *n_segments = plan->nseg;
     if (n_items) *n_items = plan->nitems;
-    if (device_bytes) *device_bytes = plan->bytes;
+    if (device_bytes) *device_bytes = plan_bytes(plan);
     return SRF_OK;
 }
 
@@ -1328,10 +1402,10 @@ int srf_plan_project(srf_plan *P, const float *image, float *out, void *stream)
     if (!image || !out) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
     if (P->nitems > 0)
-        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, P->order, image, P->G, P->T,
-                                                                       P->partial));
-    hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial, P->ev_off,
-                       P->ev_flag, P->n, (const float *)nullptr, 0, 0.0f, out);
+        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, image, P->G, P->T,
+                                                                       P->partial.p));
+    hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial.p, P->ev_off.p,
+                       P->ev_flag.p, P->n, (const float *)nullptr, 0, 0.0f, out);
     return launch_status();
 }
 
@@ -1344,10 +1418,10 @@ int srf_plan_project_ratio(srf_plan *P, const float *image, const float *y, srf_
     if (!image || !y || !q) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
     if (P->nitems > 0)
-        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, P->order, image, P->G, P->T,
-                                                                       P->partial));
-    hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial, P->ev_off,
-                       P->ev_flag, P->n, y, (int)kind, eps, q);
+        PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, image, P->G, P->T,
+                                                                       P->partial.p));
+    hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial.p, P->ev_off.p,
+                       P->ev_flag.p, P->n, y, (int)kind, eps, q);
     return launch_status();
 }
 
@@ -1357,11 +1431,11 @@ static int plan_bp(srf_plan *P, const float *values, const float *y, const float
     if (!P || !image) return SRF_ERR_INVALID_ARG;
     if (mode != SRF_BP_REFERENCE && mode != SRF_BP_ADJOINT) return SRF_ERR_INVALID_ARG;
     if (P->nitems > 0)
-        PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items, P->order, values, y, p, kind, eps,
+        PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values, y, p, kind, eps,
                                                                        (int)(mode == SRF_BP_ADJOINT), P->G, P->T,
-                                                                       P->rec_ev, P->tile_buf));
+                                                                       P->rec_ev.p, P->tile_buf.p));
     dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
-    hipLaunchKernelGGL(k_tile_reduce, grid, dim3(256), 0, s, P->tile_buf, P->tile_items, P->G, P->T, accumulate,
+    hipLaunchKernelGGL(k_tile_reduce, grid, dim3(256), 0, s, P->tile_buf.p, P->tile_items.p, P->G, P->T, accumulate,
                        image);
     return launch_status();
 }
@@ -1378,7 +1452,7 @@ int srf_plan_digest(srf_plan *P, srf_bp_mode_t mode, uint64_t *hash, int32_t *co
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, (unsigned *)count, 0u,
                        P->n);
     if (P->nitems > 0)
-        PLAN_DISPATCH(k_digest_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs, P->items,
+        PLAN_DISPATCH(k_digest_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p,
                                                                            (int)(mode == SRF_BP_ADJOINT), P->G,
                                                                            P->T, (unsigned long long *)hash, count));
     return launch_status();
@@ -1480,14 +1554,14 @@ int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int3
         ScanDesc &d = desc[(size_t)i];
         d.G = P->G;
         d.T = P->T;
-        d.recs = P->recs;
-        d.items = P->items;
-        d.rec_ev = P->rec_ev;
-        d.tile_items = P->tile_items;
-        d.ev_off = P->ev_off;
-        d.ev_flag = P->ev_flag;
-        d.partial = P->partial;
-        d.tile_buf = P->tile_buf;
+        d.recs = P->recs.p;
+        d.items = P->items.p;
+        d.rec_ev = P->rec_ev.p;
+        d.tile_items = P->tile_items.p;
+        d.ev_off = P->ev_off.p;
+        d.ev_flag = P->ev_flag.p;
+        d.partial = P->partial.p;
+        d.tile_buf = P->tile_buf.p;
         d.x = scans[i].image;
         d.emap = scans[i].emap;
         d.y = scans[i].y;
@@ -1497,7 +1571,7 @@ int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int3
         tile_base[(size_t)i + 1] = tile_base[(size_t)i] + P->T.ntiles;
         if (P->nitems > 0) {
             std::vector<Item> items((size_t)P->nitems);
-            SRF_HIP(hipMemcpyAsync(items.data(), P->items, items.size() * sizeof(Item), hipMemcpyDeviceToHost, s));
+            SRF_HIP(hipMemcpyAsync(items.data(), P->items.p, items.size() * sizeof(Item), hipMemcpyDeviceToHost, s));
             SRF_HIP(hipStreamSynchronize(s));
             for (int k = 0; k < P->nitems; ++k) {
                 work.push_back(Work{i, k});

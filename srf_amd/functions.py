@@ -423,14 +423,15 @@ class EmapGenerator:
         return Emap(emap.cpu().numpy(), list(image_config.center), list(image_config.size))
 
 
-def crystal_pair_emap(scanner, image_config, chunk_pairs=1 << 23, item_records=8192, stats=None):
+def crystal_pair_emap(scanner, image_config, chunk_pairs=1 << 24, item_records=8192, stats=None):
     """Sum over every ordered crystal pair (a, b) of the reference
     backprojection of 1 along the LOR from crystal a to crystal b, in the GPU
     kernel layout (nz, ny, nx).
 
     The crystal-centre table is built once; the pair LORs are generated on the
     GPU chunk by chunk (srf_pair_lors, chunk_pairs pairs at a time), each chunk
-    is binned into a tiled plan and backprojected into the accumulator
+    is binned into one tiled plan rebuilt in place (srf_plan_rebuild: no device
+    allocation after the first chunk) and backprojected into the accumulator
     (srf_plan_backproject with accumulate).  Self pairs and pairs whose
     transaxial extent is below 10 voxels contribute nothing, as in the
     reference backprojector (siddon.cu.cc:364)."""
@@ -447,18 +448,24 @@ def crystal_pair_emap(scanner, image_config, chunk_pairs=1 << 23, item_records=8
     chunk = int(min(chunk_pairs, total))
     lors = torch.empty((chunk, 6), dtype=torch.float32, device=dev)
     nseg = 0
+    plan = None
     with torch.cuda.device(dev):
-        for p0 in range(0, total, chunk):
-            n = min(chunk, total - p0)
-            check(lib().srf_pair_lors(pos.data_ptr(), ncrys, p0, n, lors.data_ptr(), stream_handle(dev)),
-                  "srf_pair_lors")
-            plan = ops.Plan(lors[:n], shape, geom, False, item_records=item_records)
-            try:
+        try:
+            for p0 in range(0, total, chunk):
+                n = min(chunk, total - p0)
+                check(lib().srf_pair_lors(pos.data_ptr(), ncrys, p0, n, lors.data_ptr(), stream_handle(dev)),
+                      "srf_pair_lors")
+                # one plan for every chunk: its device memory is reused
+                if plan is None:
+                    plan = ops.Plan(lors[:n], shape, geom, False, item_records=item_records)
+                else:
+                    plan.rebuild(lors[:n])
                 check(lib().srf_plan_backproject(plan.handle, None, 0, 1, acc.data_ptr(), stream_handle(dev)),
                       "srf_plan_backproject")
                 if stats is not None:
                     nseg += plan.info()["segments"]
-            finally:
+        finally:
+            if plan is not None:
                 plan.close()
     if stats is not None:
         stats.update({"crystals": ncrys, "pairs": total, "chunks": (total + chunk - 1) // chunk,

@@ -282,6 +282,20 @@ class Plan:
             raise ValueError("plan is closed")
         return int(self._h.value)
 
+    def rebuild(self, lors: Tensor, tof_values: Optional[Tensor] = None):
+        """Bin a new event list (same grid, TOF and options) into this plan,
+        reusing its device memory (srf_plan_rebuild)."""
+        n, stride, tp, ts = _lor_args(lors, tof_values, self.use_tof)
+        with torch.cuda.device(lors.device):
+            opts = SrfPlanOpts(self.item_records, self.tof_window)
+            rc = lib().srf_plan_rebuild(self.handle, lors.data_ptr(), n, stride, tp, ts, _grid(self.shape, self.geom),
+                                        _tof(self.use_tof, self.tof_bin, self.tof_sigma2), ctypes.byref(opts),
+                                        stream_handle(lors.device))
+        self.n = 0
+        check(rc, "srf_plan_rebuild")
+        self.n = n
+        return self
+
     def info(self):
         ne, ns, ni = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         nb = ctypes.c_size_t()

@@ -176,3 +176,40 @@ def test_tof_window_drops_only_negligible_terms():
     assert rel_l2(b1.cpu().numpy(), b0.cpu().numpy()) <= 1e-6
     exact.close()
     win.close()
+
+
+@pytest.mark.parametrize("tof", [False, True])
+def test_plan_rebuild_equals_fresh_plan(tof):
+    """srf_plan_rebuild reuses a plan's device memory: after rebuilding onto a
+    larger list, a smaller one and an empty one, every result is bit-identical
+    to a freshly created plan of the same events."""
+    import torch
+    from srf_amd import ops
+    from srf_amd.functions import uniform_cylinder_lors
+    shape, geom = (64, 64, 32), [0.0, 0.0, 0.0, 256.0, 256.0, 128.0]
+    tb, ts = (3.66, 649.2) if tof else (0.0, 0.0)
+
+    def lors_of(n, seed):
+        L = uniform_cylinder_lors(n, 140.0, 120.0, seed, torch.device("cuda"))
+        if tof:
+            t = (torch.rand(n, 1, device="cuda", generator=torch.Generator("cuda").manual_seed(seed)) - 0.5) * 200
+            L = torch.cat([L, t], 1).contiguous()
+        return L
+
+    img = torch.rand((32, 64, 64), device="cuda", generator=torch.Generator("cuda").manual_seed(3))
+    plan = ops.Plan(lors_of(20000, 1), shape, geom, tof, tb, ts)
+    for n, seed in ((60000, 2), (15000, 3), (0, 4), (30000, 5)):
+        L = lors_of(n, seed) if n else torch.empty((0, 7 if tof else 6), device="cuda")
+        plan.rebuild(L)
+        fresh = ops.Plan(L, shape, geom, tof, tb, ts)
+        assert plan.info()["segments"] == fresh.info()["segments"]
+        print(f"n {n}: {plan.info()}", flush=True)
+        if n:
+            assert torch.equal(plan.project(img), fresh.project(img))
+            y = torch.rand(n, device="cuda", generator=torch.Generator("cuda").manual_seed(seed)) + 0.5
+            assert torch.equal(plan.backproject(y, 0), fresh.backproject(y, 0))
+            assert torch.equal(plan.backproject(y, 1), fresh.backproject(y, 1))
+        else:
+            assert float(plan.backproject(None, 0).abs().sum()) == 0.0
+        fresh.close()
+    plan.close()
