@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--batch-scans", type=int, default=32)
     ap.add_argument("--batch-seed", type=int, default=5)
     ap.add_argument("--batch-only", action="store_true", help="only the batched many-scan reconstruction")
+    ap.add_argument("--no-emap", action="store_true", help="skip the crystal-pair sensitivity image")
     ap.add_argument("--tof-window", type=float, default=22.0,
                     help="plan TOF window (terms below 2^-w of the event's TOF peak are not binned; 0 = all terms)")
     return ap.parse_args()
@@ -226,6 +227,7 @@ def main():
     torch.cuda.empty_cache()
     scatter = None if args.no_scatter else scatter_bench(args)
     batch = None if args.no_batch else batch_bench(args)
+    emap = None if args.no_emap else emap_bench(args)
 
     out = {"metric": METRIC, "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": t / args.steps * 1e3, "higher_is_better": True,
@@ -237,7 +239,7 @@ def main():
                       "kernels": "event-per-lane" if args.event_kernels else "lds-tiled plans",
                       "tof_window": 0.0 if args.event_kernels else args.tof_window,
                       "setup_binning_s": plan_s, "plan_subset0": plan_info},
-           "roofline": roof, "cpu_baseline": cpu, "scatter": scatter, "batch": batch}
+           "roofline": roof, "cpu_baseline": cpu, "scatter": scatter, "batch": batch, "emap": emap}
     print(json.dumps(out), flush=True)
 
 
@@ -253,6 +255,49 @@ def _timed(fn):
     r = fn()
     torch.cuda.synchronize()
     return r, time.perf_counter() - t
+
+
+def emap_bench(args):
+    """Sensitivity image over every ordered crystal pair of the mCT geometry
+    (48 blocks x 4 rings of 13 x 13 crystals, 1.05e9 pairs) on its 200^3 x
+    2.05 mm image: GPU time (first call, which allocates the plan, and a second
+    one), and the CPU oracle's backprojection of a bounded sample of the same
+    pairs (--no-cpu skips it)."""
+    import numpy as np
+    import torch
+    from srf_amd import scanner as S
+    from srf_amd.data import ImageConfig
+    from srf_amd.functions import crystal_pair_emap
+    from srf_amd.scanner import crystal_positions
+    cfg = ImageConfig([200, 200, 200], [0.0, 0.0, 0.0], [410.0, 410.0, 410.0])
+    st = {}
+    e1, t1 = _timed(lambda: crystal_pair_emap(S.scanner_mct, cfg, stats=st))
+    s1 = float(e1.double().sum())
+    del e1
+    e2, t2 = _timed(lambda: crystal_pair_emap(S.scanner_mct, cfg))
+    s2 = float(e2.double().sum())
+    del e2
+    torch.cuda.empty_cache()
+    log(f"emap mCT: {st['pairs']} pairs in {t1:.2f} s / {t2:.2f} s ({st['pairs'] / t2:.3e} pairs/s)")
+    out = {"workload": "mCT (48 blocks x 4 rings, 13x13 crystals), all ordered crystal pairs, 200^3 x 2.05 mm",
+           "pairs": st["pairs"], "segments": st["segments"], "first_s": t1, "s": t2, "pairs_per_s": st["pairs"] / t2,
+           "sum_equal_between_runs": s1 == s2}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        pos = crystal_positions(S.scanner_mct)
+        n = pos.shape[0]
+        m = 1 << 20   # the first 2^20 pairs: crystals 0-32 against every crystal
+        k = np.arange(m)
+        lors = np.hstack((pos[k // n], pos[k % n])).astype(np.float32)
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        t = time.perf_counter()
+        oracle.backproject(np.ones(m, np.float32), lors, (200, 200, 200), [0.0, 0.0, 0.0], [410.0, 410.0, 410.0],
+                           f64=False, threads=threads)
+        dt = time.perf_counter() - t
+        out["cpu_oracle"] = {"pairs_per_s": m / dt, "cores": threads, "sample": f"first {m} pairs, {dt:.1f} s"}
+        log(f"emap CPU oracle: {m / dt:.3e} pairs/s on {threads} threads")
+    return out
 
 
 def scatter_bench(args):

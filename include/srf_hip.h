@@ -352,6 +352,55 @@ int srf_cnn_cast(const void *x, int64_t n, int32_t to_bf16, void *y, void *strea
 int srf_pair_lors(const float *crystal_pos, int64_t n_crystals, int64_t pair_start, int64_t n_pairs, float *lors,
                   void *stream);
 
+/* Block (ECAT) scanner as srfnef's PetEcatScanner properties
+ * (ops/geometry/pet_ecat_scanner.py:13-47); lengths in mm. */
+typedef struct {
+    double angle_per_block;   /* 2 pi / nb_blocks_per_ring */
+    double axial_length, gap;
+    double block_size_y, block_size_z;
+    double unit_y, unit_z;    /* block size / crystal grid */
+    int32_t nb_blocks_per_ring, shape_y, nb_crystals_per_block, nb_crystals_per_ring;
+} srf_ecat_t;
+/* Cylindrical scanner as PetCylindricalScanner's properties
+ * (ops/geometry/pet_cylindrical_scanner.py:13-74), [transaxial, z] pairs. */
+typedef struct {
+    double angle_per_rsector;
+    double sz_rsector[2], mv_module[2], sz_module[2], mv_submodule[2], sz_submodule[2], mv_crystal[2];
+    int32_t nb_rsector, nb_crystal[2], nb_submodule[2], nb_module[2];
+    int32_t nb_crystal_per_submodule, nb_crystal_per_module, nb_crystal_per_rsector;
+} srf_cyl_t;
+/* ind[2e], ind[2e + 1] = crystal index of LOR e's endpoints (kind 0: ECAT,
+ * EcatCrystalPosToIndex, functions/geometry/pet_ecat_scanner.py:15-62; kind 1:
+ * cylindrical, CylindricalCrystalPosToIndex, pet_cylindrical_scanner.py:15-110),
+ * in the reference's precision on float32 positions (the block / sector angle
+ * and the ECAT ring in float32, the rest in float64; NumPy rounding).  Indices
+ * may fall outside [0, nb_crystals) as the reference's do; a non-finite
+ * position gives INT64_MIN. */
+int srf_crystal_index(const float *lors, int64_t n, int64_t lor_stride, int32_t kind, const srf_ecat_t *ecat,
+                      const srf_cyl_t *cyl, int64_t *ind, void *stream);
+/* out = lors with both endpoints replaced by their crystals' centres (columns
+ * past the sixth, e.g. TOF, copied): PetEcatListmodeTrans /
+ * PetCylindricalListmodeTrans (functions/data_transform/pet_ecat.py:43-60,
+ * pet_cylindrical.py:43-60).  An index outside the table makes the LOR NaN and
+ * adds one to *outside (a device counter). */
+int srf_snap_lors(const float *lors, int64_t n, int64_t lor_stride, const int64_t *ind, const float *crystal_pos,
+                  int64_t n_crystals, float *out, uint64_t *outside, void *stream);
+/* sino[(i - 1) i / 2 + j] += value (1 if values is NULL) for each event, with
+ * (i, j) its endpoints' indices ordered i >= j: lm2sino's crystal-pair
+ * sinogram over the N (N - 1) / 2 pairs of get_all_lors_id
+ * (corrections/scattering/listmode2sinogram.py:17-35, 76-85).  Events whose
+ * bin is outside the table add one to *outside (a device counter).  Double
+ * atomics: sums of integer-valued weights are exact. */
+int srf_pair_sinogram(const int64_t *ind, const float *values, int64_t n, int64_t n_crystals, double *sino,
+                      uint64_t *outside, void *stream);
+/* sino2lm (listmode2sinogram.py:88-99): the bins with 0 < value < 1e8, in bin
+ * order, as LORs from crystal i to crystal j (6 floats) and their values.
+ * *count receives the number of such bins; with lors == NULL nothing else is
+ * written, otherwise capacity must hold *count.  Allocates its scratch;
+ * synchronises `stream`. */
+int srf_sinogram_lors(const double *sino, int64_t n_crystals, const float *crystal_pos, float *lors, double *values,
+                      int64_t capacity, int64_t *count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,23 @@ class SrfBatchCfg(ctypes.Structure):
     _fields_ = [("persistent", ctypes.c_int32), ("workgroups_per_cu", ctypes.c_int32), ("order", ctypes.c_int32)]
 
 
+class SrfEcat(ctypes.Structure):
+    _fields_ = [("angle_per_block", ctypes.c_double), ("axial_length", ctypes.c_double), ("gap", ctypes.c_double),
+                ("block_size_y", ctypes.c_double), ("block_size_z", ctypes.c_double), ("unit_y", ctypes.c_double),
+                ("unit_z", ctypes.c_double), ("nb_blocks_per_ring", ctypes.c_int32), ("shape_y", ctypes.c_int32),
+                ("nb_crystals_per_block", ctypes.c_int32), ("nb_crystals_per_ring", ctypes.c_int32)]
+
+
+class SrfCyl(ctypes.Structure):
+    _D2 = ctypes.c_double * 2
+    _I2 = ctypes.c_int32 * 2
+    _fields_ = [("angle_per_rsector", ctypes.c_double), ("sz_rsector", _D2), ("mv_module", _D2), ("sz_module", _D2),
+                ("mv_submodule", _D2), ("sz_submodule", _D2), ("mv_crystal", _D2), ("nb_rsector", ctypes.c_int32),
+                ("nb_crystal", _I2), ("nb_submodule", _I2), ("nb_module", _I2),
+                ("nb_crystal_per_submodule", ctypes.c_int32), ("nb_crystal_per_module", ctypes.c_int32),
+                ("nb_crystal_per_rsector", ctypes.c_int32)]
+
+
 class SrfError(RuntimeError):
     pass
 
@@ -67,6 +84,11 @@ SIGNATURES = [
     ("srf_lor_digest", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.c_int, _P, _P, _P]),
     ("srf_smooth", ctypes.c_int, [_P, _I32, _I32, _I32, _F, _F, _F, _P, _P, _P, _P]),
     ("srf_pair_lors", ctypes.c_int, [_P, _I64, _I64, _I64, _P, _P]),
+    ("srf_crystal_index", ctypes.c_int, [_P, _I64, _I64, _I32, ctypes.POINTER(SrfEcat), ctypes.POINTER(SrfCyl), _P,
+                                         _P]),
+    ("srf_snap_lors", ctypes.c_int, [_P, _I64, _I64, _P, _P, _I64, _P, _P, _P]),
+    ("srf_pair_sinogram", ctypes.c_int, [_P, _P, _I64, _I64, _P, _P, _P]),
+    ("srf_sinogram_lors", ctypes.c_int, [_P, _I64, _P, _P, _P, _I64, ctypes.POINTER(_I64), _P]),
     ("srf_image_to_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
     ("srf_image_from_kernel_layout", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P]),
     ("srf_plan_create", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(_P), _P]),

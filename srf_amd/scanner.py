@@ -168,6 +168,19 @@ class PetCylindricalScanner:
         return self.nb_crystal_per_rsector * self.nb_rsector
 
 
+def _positions(pos):
+    """(N, 3) positions in their own floating type.  Listmode LORs are float32,
+    and the reference's arithmetic on them stays float32 where no float64 array
+    enters (the block / sector angle from arctan2, the ring from the axial
+    coordinate); the in-block corrections are float64."""
+    pos = np.asarray(pos)
+    if pos.dtype.kind != "f":
+        pos = pos.astype(np.float64)
+    if pos.ndim == 1:
+        pos = pos.reshape((1, pos.size))
+    return pos
+
+
 # ----------------------------------------------------------------- ECAT --
 
 @nef_class
@@ -208,9 +221,7 @@ class EcatCrystalPosToIndex:
     def __call__(self, pos, verbose=False):
         sc = self.scanner
         b = sc.blocks
-        pos = np.asarray(pos, dtype=np.float64)
-        if pos.ndim == 1:
-            pos = pos.reshape((1, pos.size))
+        pos = _positions(pos)
         xc, yc, zc = pos[:, 0], pos[:, 1], pos[:, 2]
         iblock = np.round(np.arctan2(yc, xc) / sc.angle_per_block).astype(int) % sc.nb_blocks_per_ring
         iring = np.floor((zc + sc.axial_length / 2) / (sc.gap + b.size[2])).astype(int)
@@ -274,13 +285,11 @@ class CylindricalCrystalPosToIndex:
 
     def __call__(self, pos, verbose=False):
         sc = self.scanner
-        pos = np.asarray(pos, dtype=np.float64)
-        if pos.ndim == 1:
-            pos = pos.reshape((1, pos.size))
+        pos = _positions(pos)
         rs = np.round(np.arctan2(pos[:, 1], pos[:, 0]) / sc.angle_per_rsector).astype(int) % sc.nb_rsector
         angle = rs * sc.angle_per_rsector
         y = -pos[:, 0] * np.sin(angle) + pos[:, 1] * np.cos(angle)
-        z = pos[:, 2].copy()
+        z = pos[:, 2].astype(np.float64)
         my = np.floor((y + sc.sz_rsector[0] / 2) / sc.mv_module[0]).astype(int)
         mz = np.floor((z + sc.sz_rsector[1] / 2) / sc.mv_module[1]).astype(int)
         y = y - (my * sc.mv_module[0] - sc.sz_rsector[0] / 2 + sc.sz_module[0] / 2)

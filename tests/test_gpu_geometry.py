@@ -94,3 +94,108 @@ def test_emap_quarter_turn_symmetry():
     err = rel_l2(rot, em)
     print(f"quarter-turn asymmetry {err:.2e}", flush=True)
     assert err <= 2e-2
+
+
+def _jittered(sc, n, seed):
+    """Endpoints drawn on random crystals, moved by up to 0.45 crystal pitch
+    in-plane (along the block face) and axially: each stays inside its crystal."""
+    from srf_amd.scanner import PetEcatScanner, crystal_positions
+    rng = np.random.default_rng(seed)
+    pos = crystal_positions(sc).astype(np.float64)
+    if isinstance(sc, PetEcatScanner):
+        uy, uz = sc.blocks.unit_size[1], sc.blocks.unit_size[2]
+    else:
+        uy, uz = sc.sz_crystal[0], sc.sz_crystal[1]
+    ends = []
+    ids = []
+    for _ in range(2):
+        k = rng.integers(0, pos.shape[0], n)
+        p = pos[k].copy()
+        r = np.hypot(p[:, 0], p[:, 1])
+        # tangent at the crystal: perpendicular to its block normal, which is the
+        # direction of the crystal centre from the axis minus the in-face offset
+        t = np.stack([-p[:, 1], p[:, 0]], 1) / r[:, None]
+        du = rng.uniform(-0.45, 0.45, n) * uy
+        dz = rng.uniform(-0.45, 0.45, n) * uz
+        # moving along the true face tangent keeps the block normal distance; use
+        # a small step so the chord error stays far below the crystal pitch
+        p[:, 0] += 0.2 * du * t[:, 0]
+        p[:, 1] += 0.2 * du * t[:, 1]
+        p[:, 2] += dz
+        ends.append(p)
+        ids.append(k)
+    lors = np.hstack(ends).astype(np.float32)
+    return lors, np.stack(ids, 1)
+
+
+@pytest.mark.parametrize("name", ["scanner_mct", "scanner_8panel"])
+def test_crystal_indices_and_snap(name):
+    import torch
+    from srf_amd import scanner as S
+    from srf_amd.data_transform import crystal_indices, snap_lors
+    from srf_amd.scanner import crystal_pos_to_index, crystal_positions
+    sc = getattr(S, name)
+    pos = crystal_positions(sc)
+    # every crystal centre maps to its own index
+    both = np.hstack((pos, pos[::-1].copy()))
+    ind = crystal_indices(sc, both).cpu().numpy()
+    assert np.array_equal(ind[:, 0], np.arange(sc.nb_crystals))
+    assert np.array_equal(ind[:, 1], np.arange(sc.nb_crystals)[::-1])
+    # endpoints moved inside their crystals keep the index; the host map (the
+    # reference's float32 / float64 arithmetic) agrees on every endpoint
+    lors, ids = _jittered(sc, 200000, 11)
+    g = crystal_indices(sc, lors).cpu().numpy()
+    assert np.array_equal(g, ids)
+    host = np.stack([crystal_pos_to_index(sc, lors[:, :3]), crystal_pos_to_index(sc, lors[:, 3:6])], 1)
+    assert np.array_equal(g, host)
+    # snapping: crystal centres, TOF column kept, off-scanner endpoints -> NaN
+    tof = np.random.default_rng(2).normal(0, 50, (lors.shape[0], 1)).astype(np.float32)
+    L7 = np.hstack((lors, tof))
+    L7[:5, 2] = 1e4                       # far outside the axial extent
+    out, outside = snap_lors(sc, L7)
+    out = out.cpu().numpy()
+    assert outside == 5 and np.isnan(out[:5, :6]).all()
+    assert np.array_equal(out[5:, :3], pos[ids[5:, 0]]) and np.array_equal(out[5:, 3:6], pos[ids[5:, 1]])
+    assert np.array_equal(out[:, 6], L7[:, 6])
+
+
+def test_pair_sinogram_round_trip():
+    import torch
+    from srf_amd.data import Listmode, Lors
+    from srf_amd.data_transform import get_all_lors_id, lm2sino, sino2lm
+    from srf_amd.scanner import crystal_positions
+    sc = small_ecat()
+    n = sc.nb_crystals
+    pos = crystal_positions(sc)
+    rng = np.random.default_rng(4)
+    a, b = rng.integers(0, n, 50000), rng.integers(0, n, 50000)
+    a[:7] = b[:7] = 3                     # self pairs: the reference's next-row bin
+    a[7] = b[7] = n - 1                   # the last self pair falls off the table
+    w = rng.integers(1, 5, 50000).astype(np.float32)
+    lm = Listmode(w, Lors(np.hstack((pos[a], pos[b])).astype(np.float32)))
+    sino, outside = lm2sino(lm, sc, return_outside=True)
+    s = sino.cpu().numpy()
+    # the reference's rule per event: (i, j) = (max, min); a self pair i == j
+    # lands in bin i (i + 1) / 2, pair (i + 1, 0), off the table for i = N - 1
+    hi, lo = np.maximum(a, b), np.minimum(a, b)
+    off = (hi == lo) & (hi == n - 1)
+    assert outside == int(off.sum()) and off[7]
+    assert s.shape == (n * (n - 1) // 2,)
+    assert s.sum() == float(w.sum() - w[off].sum())          # integer weights: exact
+    pairs = get_all_lors_id(n)
+    expect = {}
+    for i, j, v, o in zip(hi, lo, w, off):
+        if o:
+            continue
+        key = (int(i) + 1, 0) if i == j else (int(i), int(j))
+        expect[key] = expect.get(key, 0.0) + float(v)
+    nz = np.nonzero(s)[0]
+    got = {(int(pairs[k, 0]), int(pairs[k, 1])): float(s[k]) for k in nz}
+    assert got == pytest.approx(expect)
+    assert tuple(pairs[3 * 4 // 2]) == (4, 0)                # crystal 3's self pair
+    # back to LORs: bins in order, crystal i then crystal j, value
+    out = sino2lm(sc, sino)
+    assert out.shape == (len(nz), 7)
+    np.testing.assert_array_equal(out[:, :3], pos[pairs[nz, 0]])
+    np.testing.assert_array_equal(out[:, 3:6], pos[pairs[nz, 1]])
+    np.testing.assert_array_equal(out[:, 6], s[nz])

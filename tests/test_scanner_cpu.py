@@ -99,3 +99,16 @@ def test_cylindrical_module_stride_quirk():
     got = S.CylindricalCrystalPosToIndex(sc)(p)
     assert got[0] == 1 * 3 * sc.nb_crystal_per_module             # mz * nb_module[1]
     assert got[0] != 1 * 2 * sc.nb_crystal_per_module             # what the forward map uses
+
+
+def test_all_lors_id_order():
+    """Pairs (i, j), i > j, row by row: bin (i - 1) i / 2 + j, N (N - 1) / 2 bins
+    (listmode2sinogram.py:27-35)."""
+    from srf_amd.data_transform import get_all_lors_id
+    n = 37
+    g = get_all_lors_id(n)
+    assert g.shape == (n * (n - 1) // 2, 2) and g.dtype == np.int32
+    assert (g[:, 0] > g[:, 1]).all() and (g[:, 1] >= 0).all()
+    b = (g[:, 0].astype(np.int64) - 1) * g[:, 0] // 2 + g[:, 1]
+    assert np.array_equal(b, np.arange(g.shape[0]))
+    assert g[:6].tolist() == [[1, 0], [2, 0], [2, 1], [3, 0], [3, 1], [3, 2]]
