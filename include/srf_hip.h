@@ -132,6 +132,14 @@ int srf_lor_digest(const float *lors, int64_t n, int64_t lor_stride,
 int srf_smooth(const float *x, int32_t n0, int32_t n1, int32_t n2, float sigma0, float sigma1, float sigma2,
                const float *mask, float *tmp, float *out, void *stream);
 
+/* In-plane PSF blur of x (nx ny, nz; slices fastest) with the sparse K_xy of
+ * srfnef's Deconvolute (corrections/psf/deconvolute.py:56-62, built by
+ * xy_main :163-213) held as per-source-pixel patch weights
+ * weights[p][(a + half) (2 half + 1) + b + half] on target pixel p + (a, b).
+ * adjoint 0: out = K_xy x, 1: out = K_xy^T x; d != NULL: out = d / (. + eps).
+ * out must not alias x or d. */
+int srf_psf_xy(const float *x, int32_t nx, int32_t ny, int32_t nz, const float *weights, int32_t half,
+               int32_t adjoint, const float *d, float eps, float *out, void *stream);
 /* Layout conversions between the Python (nx,ny,nz) C-order image (z fastest)
  * and the kernel layout (x fastest): dst = transpose(src). */
 int srf_image_to_kernel_layout(const float *src_xyz, float *dst_kernel, int32_t nx, int32_t ny,

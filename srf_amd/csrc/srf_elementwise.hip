@@ -133,6 +133,40 @@ __global__ __launch_bounds__(256) void k_gauss_pass(const float *__restrict__ x,
     out[i] = v;
 }
 
+// PSF blur in the plane (corrections/psf/deconvolute.py:56-62): the sparse
+// K_xy kept as each source pixel's (2 half + 1)^2 patch weights w[p][(a + half)
+// (2 half + 1) + b + half] on target pixel p + (a, b); x is (nx ny, nz), slices
+// fastest.  adjoint 0: out[q] = sum_o w[q - o][o] x[q - o] (K_xy x), adjoint 1:
+// out[p] = sum_o w[p][o] x[p + o] (K_xy^T x); with d, out = d / (sum + eps).
+// One thread per (pixel, slice): the patch weights are shared by a pixel's
+// slices, the x reads coalesce along the slices.
+__global__ __launch_bounds__(256) void k_psf_xy(const float *__restrict__ x, int nx, int ny, int nz,
+                                                const float *__restrict__ w, int half, int adjoint,
+                                                const float *__restrict__ d, float eps, float *__restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (int64_t)nx * ny * nz) return;
+    const int z = (int)(t % nz);
+    const int64_t p = t / nz;
+    const int i = (int)(p / ny), j = (int)(p % ny);
+    const int k = 2 * half + 1;
+    float s = 0.0f;
+    for (int a = -half; a <= half; ++a) {
+        const int ii = adjoint ? i + a : i - a;
+        if (ii < 0 || ii >= nx) continue;
+        for (int b = -half; b <= half; ++b) {
+            const int jj = adjoint ? j + b : j - b;
+            if (jj < 0 || jj >= ny) continue;
+            const int64_t q = (int64_t)ii * ny + jj;
+            const int o = (a + half) * k + (b + half);
+            const float wt = adjoint ? w[p * k * k + o] : w[q * k * k + o];
+            s = fmaf(wt, x[q * nz + z], s);
+        }
+    }
+    if (d) s = d[t] / (s + eps);
+    out[t] = s;
+}
+
 }  // namespace srf
 
 using namespace srf;
@@ -209,6 +243,18 @@ int srf_image_from_kernel_layout(const float *src_kernel, float *dst_xyz, int32_
                                  void *stream)
 {
     return swap_outer(src_kernel, dst_xyz, nz, ny, nx, (hipStream_t)stream);
+}
+
+int srf_psf_xy(const float *x, int32_t nx, int32_t ny, int32_t nz, const float *weights, int32_t half,
+               int32_t adjoint, const float *d, float eps, float *out, void *stream)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0) return SRF_ERR_INVALID_ARG;
+    if (!x || !weights || !out || half < 0 || half > 32 || (adjoint != 0 && adjoint != 1)) return SRF_ERR_INVALID_ARG;
+    if (x == out || (d && d == out)) return SRF_ERR_INVALID_ARG;
+    const int64_t n = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_psf_xy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x, nx, ny, nz,
+                       weights, half, adjoint, d, eps, out);
+    return launch_status();
 }
 
 }  // extern "C"
