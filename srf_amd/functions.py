@@ -202,7 +202,12 @@ class Mlem:
 
 def _mlem_full(self, tof):
     if self.emap is None:
-        object.__setattr__(self, "emap", EmapGenerator("uniform", self.scanner)(self.image_config))
+        # mlem_full.py:39-41: EmapGenerator('block-full', scanner); an ideal
+        # cylinder (no crystals) gets the sampled sensitivity image
+        from .scanner import PetCylindricalScanner, PetEcatScanner
+        crystals = isinstance(self.scanner, (PetEcatScanner, PetCylindricalScanner))
+        object.__setattr__(self, "emap", EmapGenerator("block-full" if crystals else "uniform",
+                                                       self.scanner)(self.image_config))
     lm = self.listmode
     if self.atten_corr is not None:
         lm = self.atten_corr(lm)

@@ -1,0 +1,62 @@
+"""Recon descriptions (srf_amd.recon) on the CPU: the JSON schema of srfnef's
+recon documents read into scanner / image / listmode objects, array loading
+without pickles, and the refusal of HDF5 stores."""
+import json
+
+import numpy as np
+import pytest
+
+from srf_amd import recon, scanner as S
+
+
+MCT = {"inner_radius": 424.5, "outer_radius": 444.5, "nb_rings": 4, "nb_blocks_per_ring": 48, "gap": 0.0,
+       "blocks": {"size": [20.0, 53.3, 53.3], "shape": [1, 13, 13], "interval": [0, 0, 0]}, "tof": None,
+       "center": [0.0, 0.0, 0.0]}
+
+
+def test_scanner_from_dict():
+    sc = recon.scanner_from_dict(MCT)
+    ref = S.scanner_mct
+    for k in ("inner_radius", "outer_radius", "nb_rings", "nb_blocks_per_ring", "gap"):
+        assert getattr(sc, k) == getattr(ref, k)
+    assert sc.blocks.size == ref.blocks.size and sc.blocks.shape == ref.blocks.shape
+    assert np.array_equal(S.crystal_positions(sc), S.crystal_positions(ref))
+    cyl = recon.scanner_from_dict({"inner_radius": 146.7, "outer_radius": 166.7, "nb_rsector": 8,
+                                   "nb_module": [2, 2], "mv_module": [60, 80], "nb_submodule": [3, 4],
+                                   "mv_submodule": [20, 20], "nb_crystal": [6, 6], "mv_crystal": [3.2, 3.2],
+                                   "sz_crystal": [3, 3], "tof": {"tof_bin": 3.66, "tof_sigma2": 649.2}})
+    assert cyl.nb_crystals == S.scanner_8panel.nb_crystals and cyl.tof.tof_sigma2 == 649.2
+    with pytest.raises(ValueError):
+        recon.scanner_from_dict({"inner_radius": 1.0})
+
+
+def test_load_array_and_listmode(tmp_path):
+    a = np.arange(12, dtype=np.float32).reshape(2, 6)
+    np.save(tmp_path / "l.npy", a)
+    np.savez(tmp_path / "lm.npz", data=np.array([2.0, 3.0], np.float32), lors=a)
+    assert np.array_equal(recon.load_array(str(tmp_path / "l.npy")), a)
+    lm = recon.listmode_from(str(tmp_path / "lm.npz"))
+    assert np.array_equal(lm.lors.data, a) and lm.data.tolist() == [2.0, 3.0]
+    lm = recon.listmode_from(str(tmp_path / "l.npy"))
+    assert lm.data.tolist() == [1.0, 1.0]
+    with pytest.raises(NotImplementedError):
+        recon.load_array(str(tmp_path / "x.hdf5"))
+    # object arrays (pickles) are refused
+    np.save(tmp_path / "o.npy", np.array([{"a": 1}], dtype=object), allow_pickle=True)
+    with pytest.raises(ValueError):
+        recon.load_array(str(tmp_path / "o.npy"))
+
+
+def test_recon_from_json(tmp_path):
+    lors = np.zeros((4, 6), np.float32)
+    np.save(tmp_path / "lors.npy", lors)
+    cfg = {"n_iter": 3, "image_config": {"shape": [8, 8, 4], "center": [0.0, 0.0, 0.0], "size": [16.0, 16.0, 8.0]},
+           "scanner": MCT, "listmode": str(tmp_path / "lors.npy"), "atten_corr": None, "scatter_corr": None,
+           "psf_corr": None, "emap": None}
+    path = tmp_path / "recon.json"
+    path.write_text(json.dumps(cfg))
+    m = recon.recon_from_dict(recon.json_load(str(path)))
+    assert type(m).__name__ == "MlemFull" and m.n_iter == 3 and m.emap is None
+    assert m.image_config.shape == [8, 8, 4] and m.scanner.nb_crystals == S.scanner_mct.nb_crystals
+    cfg["scanner"] = dict(MCT, tof={"tof_bin": 3.66, "tof_sigma2": 649.2})
+    assert type(recon.recon_from_dict(cfg)).__name__ == "MlemFullTof"
