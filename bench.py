@@ -159,13 +159,15 @@ def main():
         if timed:
             b.record(stream)
         if plan is not None:
-            bp = plan.backproject(q, 0)
+            # backprojection with its tile sums fused into the EM update
+            plan.backproject_update(q, eng.x, eng.emap_k, 1e8)
         else:
             bp = ops.backproject_ratio(lors_s, y_s, p, 1, 1e-5, eng.shape, eng.geom, True, tof_bin, tof_sigma2, 0)
         if timed:
             c.record(stream)
             ev.append((a, b, c))
-        ops.em_update(eng.x, bp, eng.emap_k, 1e8)
+        if plan is None:
+            ops.em_update(eng.x, bp, eng.emap_k, 1e8)
 
     def step(timed):
         for k in range(eng.n_sub):

@@ -209,6 +209,11 @@ int srf_plan_project_ratio(srf_plan *plan, const float *image, const float *y, s
 int srf_plan_backproject(srf_plan *plan, const float *values, srf_bp_mode_t mode,
                          int accumulate, float *image, void *stream);
 /* image (+)= backprojection of ratio(y, p) (srf_backproject_ratio semantics). */
+/* One EM update from a plan: x = x * srf_plan_backproject(values, reference
+ * mode) / emap (emap 0 -> zero_guard when it is not 0), the tile sums fused
+ * with srf_em_update's operations (bit-identical to the two calls). */
+int srf_plan_backproject_update(srf_plan *plan, const float *values, float *x, const float *emap,
+                                float zero_guard, void *stream);
 int srf_plan_backproject_ratio(srf_plan *plan, const float *y, const float *p,
                                srf_ratio_kind_t kind, float eps, srf_bp_mode_t mode,
                                int accumulate, float *image, void *stream);

@@ -110,6 +110,7 @@ SIGNATURES = [
                                      ctypes.POINTER(ctypes.c_size_t)]),
     ("srf_plan_project", ctypes.c_int, [_P, _P, _P, _P]),
     ("srf_plan_backproject", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, _P, _P]),
+    ("srf_plan_backproject_update", ctypes.c_int, [_P, _P, _P, _P, _F, _P]),
     ("srf_plan_backproject_ratio", ctypes.c_int, [_P, _P, _P, ctypes.c_int, _F, ctypes.c_int, ctypes.c_int, _P,
                                                   _P]),
     ("srf_plan_digest", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P]),

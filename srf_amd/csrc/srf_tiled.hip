@@ -56,6 +56,15 @@ constexpr int TVOX = TX * TY * TZ;      // 8192 voxels
 constexpr int LSY = TX + 1, LSZ = LSY * TY + 1;
 constexpr int TPAD = LSZ * (TZ - 1) + LSY * (TY - 1) + TX;   // 8454 words: 33 KiB (float) / 66 KiB (fixed point)
 __device__ __forceinline__ int tile_word(int i) { return (i & (TX - 1)) + LSY * ((i / TX) & (TY - 1)) + LSZ * (i / (TX * TY)); }
+// The projector's tile carries a one-cell zero halo on every side: cell
+// (lx, ly, lz), -1 <= l <= T, at word (lx + 1) + HSY (ly + 1) + HSZ (lz + 1).
+// A gathered cell is clamped into [-1, in-image extent] per axis instead of
+// being tested against the tile: every cell outside the tile's in-image part
+// then reads a zero (its term belongs to another tile's segment or to none).
+// Odd strides keep the planes of one primary step spread over the banks.
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+constexpr int HSY = HX + 1, HSZ = HSY * HY + 1;
+constexpr int HPAD = HSZ * (HZ - 1) + HSY * (HY - 1) + HX;   // 11908 words: 47 KiB
 constexpr int NCLASS = 4;               // slab-count classes per tile
 constexpr int ITEM = 8192;              // records per work item (at most)
 constexpr int MAX_KEYS = 16384;         // LDS histogram of the binning kernels (64 KiB): up to 4096 tiles
@@ -341,6 +350,22 @@ __device__ __forceinline__ unsigned long long fixed_term(double v, float r)
     return (unsigned long long)__double_as_longlong(s) - 0x4338000000000000ull;
 }
 
+// med3(x, lo, hi) = min(max(x, lo), hi) for lo <= hi, one v_med3_i32 (the
+// compiler emits it only for constant bounds), and a * b + c on the low 24
+// bits of a and b (v_mad_u32_u24).  Plain ALU instructions, no memory access.
+__device__ __forceinline__ int med3_i32(int x, int lo, int hi)
+{
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+    return r;
+}
+__device__ __forceinline__ int mad_u24(int a, int b, int c)
+{
+    int r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // Minimum / maximum of two non-negative finite floats as integer operations on
 // their bit patterns (which order the same way): no NaN canonicalisation.
 __device__ __forceinline__ float min_nonneg(float a, float b)
@@ -395,7 +420,9 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
     const float np2 = XD ? G.nx2 : G.ny2;
     const int ns = XD ? G.ny : G.nx, nz = G.nz;
     const int p0 = XD ? t.x0 : t.y0, s0 = XD ? t.y0 : t.x0, z0 = t.z0;
-    constexpr int SP = XD ? 1 : LSY, SS = XD ? LSY : 1;   // LDS strides of primary / secondary
+    constexpr bool HALO = MODE == M_GATHER;                // the projector's halo tile
+    constexpr int RS = HALO ? HSY : LSY, ZS = HALO ? HSZ : LSZ;
+    constexpr int SP = XD ? 1 : RS, SS = XD ? RS : 1;      // LDS strides of primary / secondary
     const bool ks_pos = ks >= 0.0f, kz_pos = kz >= 0.0f;
     // in-image extent of the tile along secondary / axial
     const unsigned es = (unsigned)min(TX, ns - s0), ez = (unsigned)min(TZ, nz - z0);
@@ -410,9 +437,14 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
     // LDS byte address of cell (c_s, c_z) of slab ip: base + c_s * BS + c_z *
     // BZ, the tile origin folded into base
     constexpr int ES = (MODE == M_SCATTER_FX) ? 8 : 4;
-    constexpr unsigned BS = SS * ES, BZ = LSZ * ES;   // byte strides of secondary / axial
-    int base = (a - p0) * SP * ES - s0 * (int)BS - z0 * (int)BZ;
+    constexpr unsigned BS = SS * ES, BZ = ZS * ES;    // byte strides of secondary / axial
+    int base = HALO ? (a - p0 + 1) * SP * ES - (s0 - 1) * (int)BS - (z0 - 1) * (int)BZ
+                    : (a - p0) * SP * ES - s0 * (int)BS - z0 * (int)BZ;
     constexpr int bstep = SP * ES;
+    // halo clamp bounds and byte strides as registers (a VOP3 instruction reads
+    // at most one scalar register on this target)
+    const int slo = s0 - 1, shi = s0 + (int)es, zlo = z0 - 1, zhi = z0 + (int)ez;
+    const int bs_v = (int)BS, bz_v = (int)BZ;
     // rejected terms: a zero word (gathers, never written), the lane's own
     // slot after the fixed-point tile (fixed-point scatters), or an address
     // no tile cell has (float scatters and the digest test for it)
@@ -441,19 +473,35 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const int cz1 = cvt_flr(zl), cz2 = cvt_flr(zh);
         const bool bs = cs1 != cs2, bz = cz1 != cz2;
         // LDS addresses of the three terms (dummy where the term is not
-        // applied); term 2 is i2a if rs > rz, else i2b
-        const bool ts1 = (unsigned)(cs1 - s0) < es, ts2 = (unsigned)(cs2 - s0) < es;
-        const bool tz1 = (unsigned)(cz1 - z0) < ez, tz2 = (unsigned)(cz2 - z0) < ez;
+        // applied); term 2 is i2a if rs > rz, else i2b.  (24-bit
+        // multiply-adds: a cell whose address is used lies in the tile or its
+        // halo, so its coordinates are below 2^24; logical operators keep the
+        // predicates as lane masks.)
         const bool los = cs1 >= -1, loz = cz1 >= -1;
-        // (24-bit multiply-adds: a cell whose address is used lies in the tile,
-        // so its coordinates are below 2^24; logical operators keep the
-        // predicates as lane masks)
-        const int A = (int)(__umul24((unsigned)cs1, BS) + (unsigned)base);
-        const int B = (int)(__umul24((unsigned)cs2, BS) + (unsigned)base);
-        const int i1 = (ts1 && tz1) ? (int)(__umul24((unsigned)cz1, BZ) + (unsigned)A) : dummy;
-        const int i3 = ((bs || bz) && ts2 && tz2 && los && loz) ? (int)(__umul24((unsigned)cz2, BZ) + (unsigned)B) : dummy;
-        const int i2a = (bs && bz && ts1 && tz2 && loz) ? (int)(__umul24((unsigned)cz2, BZ) + (unsigned)A) : dummy;
-        const int i2b = (bs && bz && ts2 && tz1 && los) ? (int)(__umul24((unsigned)cz1, BZ) + (unsigned)B) : dummy;
+        int i1, i2a, i2b, i3;
+        if (HALO) {
+            // clamped into the halo: the tile tests become four med3, each
+            // address two 24-bit multiply-adds
+            const int c1 = med3_i32(cs1, slo, shi), c2 = med3_i32(cs2, slo, shi);
+            const int d1 = med3_i32(cz1, zlo, zhi), d2 = med3_i32(cz2, zlo, zhi);
+            const int A = mad_u24(c1, bs_v, base), B = mad_u24(c2, bs_v, base);
+            // computed before the selects: inline asm is not speculated, so a
+            // multiply-add inside a select would become a branch
+            const int j3 = mad_u24(d2, bz_v, B), j2a = mad_u24(d2, bz_v, A), j2b = mad_u24(d1, bz_v, B);
+            i1 = mad_u24(d1, bz_v, A);
+            i3 = ((bs || bz) && los && loz) ? j3 : dummy;
+            i2a = (bs && bz && loz) ? j2a : dummy;
+            i2b = (bs && bz && los) ? j2b : dummy;
+        } else {
+            const bool ts1 = (unsigned)(cs1 - s0) < es, ts2 = (unsigned)(cs2 - s0) < es;
+            const bool tz1 = (unsigned)(cz1 - z0) < ez, tz2 = (unsigned)(cz2 - z0) < ez;
+            const int A = (int)(__umul24((unsigned)cs1, BS) + (unsigned)base);
+            const int B = (int)(__umul24((unsigned)cs2, BS) + (unsigned)base);
+            i1 = (ts1 && tz1) ? (int)(__umul24((unsigned)cz1, BZ) + (unsigned)A) : dummy;
+            i3 = ((bs || bz) && ts2 && tz2 && los && loz) ? (int)(__umul24((unsigned)cz2, BZ) + (unsigned)B) : dummy;
+            i2a = (bs && bz && ts1 && tz2 && loz) ? (int)(__umul24((unsigned)cz2, BZ) + (unsigned)A) : dummy;
+            i2b = (bs && bz && ts2 && tz1 && los) ? (int)(__umul24((unsigned)cz1, BZ) + (unsigned)B) : dummy;
+        }
         // materialise the addresses here: sunk past the branch below, their
         // lane-mask operands would be carried across it as 0/1 vectors
         asm volatile("" ::"v"(i1), "v"(i2a), "v"(i2b), "v"(i3));
@@ -582,8 +630,8 @@ __device__ __forceinline__ void walk_for(TileWalk &t, float *lds, int dummy, int
 // batched backprojector would otherwise use more).
 #define BP_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8)))
 
-// Projector LDS: the tile plus one zero word, the target of rejected terms.
-constexpr int LDS_WORDS = TPAD + 1;
+// Projector LDS: the halo tile plus one zero word, the target of rejected terms.
+constexpr int LDS_WORDS = HPAD + 1;
 
 // Projection of one work item: stage its tile in LDS, march every record,
 // one partial sum per record.
@@ -594,11 +642,15 @@ __device__ __forceinline__ void fp_item(const SegRec *__restrict__ recs, const I
 {
     int x0, y0, z0;
     tile_origin(it.tile, T, x0, y0, z0);
-    for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) {
-        const int gx = x0 + (i % TX), gy = y0 + (i / TX) % TY, gz = z0 + i / (TX * TY);
-        tile[tile_word(i)] = (gx < G.nx && gy < G.ny && gz < G.nz) ? img[gx + G.nx * (gy + G.ny * gz)] : 0.0f;
+    for (int i = threadIdx.x; i < HX * HY * HZ; i += TILE_THREADS) {
+        const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+        const int lx = hx - 1, ly = hy - 1, lz = hz - 1;
+        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        const bool core = (unsigned)lx < (unsigned)TX && (unsigned)ly < (unsigned)TY && (unsigned)lz < (unsigned)TZ;
+        tile[hx + HSY * hy + HSZ * hz] =
+            (core && gx < G.nx && gy < G.ny && gz < G.nz) ? img[gx + G.nx * (gy + G.ny * gz)] : 0.0f;
     }
-    if (threadIdx.x == 0) tile[TPAD] = 0.0f;
+    if (threadIdx.x == 0) tile[HPAD] = 0.0f;
     __syncthreads();
     // the next record is loaded while this one is marched
     const int end = it.start + it.count;
@@ -609,7 +661,7 @@ __device__ __forceinline__ void fp_item(const SegRec *__restrict__ recs, const I
         const SegRec R = Rn;
         if (r + TILE_THREADS < end) Rn = recs[r + TILE_THREADS];
         TileWalk t;
-        walk_for(t, tile, TPAD, x0, y0, z0);
+        walk_for(t, tile, HPAD, x0, y0, z0);
         march_any<TOF, DS, DZ, M_GATHER>(R, G, (int)(R.ab_fp & 0xffffu), (int)(R.ab_fp >> 16), R.Wfp, t);
         partial[R.slot] = t.acc;
     }
@@ -807,6 +859,27 @@ __global__ __launch_bounds__(256) void k_tile_reduce(const float *__restrict__ t
     img[v] = accumulate ? img[v] + s : s;
 }
 
+// x[v] = x[v] * (sum over v's tile items) / emap[v] (emap 0 -> guard when
+// guard != 0): k_tile_reduce fused with srf_em_update, the same operations.
+__global__ __launch_bounds__(256) void k_tile_update(const float *__restrict__ tile_out,
+                                                     const int *__restrict__ tile_items, LaunchGeom G, TileCfg T,
+                                                     const float *__restrict__ emap, float guard,
+                                                     float *__restrict__ x)
+{
+    const int tile = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= TVOX) return;
+    int x0, y0, z0;
+    tile_origin(tile, T, x0, y0, z0);
+    const int gx = x0 + (i % TX), gy = y0 + (i / TX) % TY, gz = z0 + i / (TX * TY);
+    if (gx >= G.nx || gy >= G.ny || gz >= G.nz) return;
+    const float s = sum_items(tile_out, tile_items[tile], tile_items[tile + 1], i);
+    const int64_t v = gx + (int64_t)G.nx * (gy + (int64_t)G.ny * gz);
+    float e = emap[v];
+    if (guard != 0.0f && e == 0.0f) e = guard;
+    x[v] = x[v] * s / e;
+}
+
 // Per item, the largest weight constant of its records in each mode.
 __global__ __launch_bounds__(256) void k_item_wmax(const SegRec *__restrict__ recs, Item *__restrict__ items)
 {
@@ -835,15 +908,33 @@ __global__ __launch_bounds__(256) void k_item_wmax(const SegRec *__restrict__ re
 
 // p[e] = sum of e's segment partials (in segment order); NaN for non-finite LORs.
 // With y != null the EM ratio is applied on the way out: out[e] = ratio(y[e], p[e]).
+// The 256 events of a block own one contiguous range of partials: it is
+// staged in LDS with coalesced loads (when it fits), then every thread sums
+// its event's partials in order -- the same additions as a direct loop.
+constexpr int SEG_STAGE = 6144;   // partials per block staged (24 KiB; ~2300 at 9 segments per event)
+
 __global__ __launch_bounds__(256) void k_seg_reduce(const float *__restrict__ partial, const int *__restrict__ ev_off,
                                                     const uint8_t *__restrict__ ev_flag, int64_t n,
                                                     const float *__restrict__ y, int kind, float eps,
                                                     float *__restrict__ out)
 {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    __shared__ float st[SEG_STAGE];
+    const int64_t e0 = (int64_t)blockIdx.x * 256;
+    const int64_t e = e0 + threadIdx.x;
+    const int64_t eb = min(e0 + 256, n);
+    const int s0 = ev_off[e0], s1 = ev_off[eb];
+    const bool staged = s1 - s0 <= SEG_STAGE;
+    if (staged) {
+        for (int k = s0 + (int)threadIdx.x; k < s1; k += 256) st[k - s0] = partial[k];
+        __syncthreads();
+    }
     if (e >= n) return;
     float s = 0.0f;
-    for (int k = ev_off[e]; k < ev_off[e + 1]; ++k) s += partial[k];
+    if (staged) {
+        for (int k = ev_off[e]; k < ev_off[e + 1]; ++k) s += st[k - s0];
+    } else {
+        for (int k = ev_off[e]; k < ev_off[e + 1]; ++k) s += partial[k];
+    }
     if (ev_flag[e]) s = __builtin_nanf("");
     if (y) {
         const float d = s + eps;
@@ -1075,12 +1166,16 @@ void dfree(DevBuf<T> &b)
     b.cap = 0;
 }
 
-// At least `count` elements (1 if count is 0), reallocated only to grow.
+// At least `count` elements (1 if count is 0), reallocated only to grow.  A
+// buffer that grows again (a rebuild with more segments than any before) gets
+// a quarter of headroom, so that a sequence of similar rebuilds reallocates
+// (and synchronises in hipFree) a few times rather than at every new maximum.
 template <class T>
 int ensure(DevBuf<T> &b, size_t count)
 {
     if (count == 0) count = 1;
     if (b.cap >= count) return SRF_OK;
+    if (b.cap > 0) count += count / 4;
     dfree(b);
     if (hipMalloc((void **)&b.p, count * sizeof(T)) != hipSuccess) {
         (void)hipGetLastError();
@@ -1462,6 +1557,21 @@ int srf_plan_backproject(srf_plan *P, const float *values, srf_bp_mode_t mode, i
                          void *stream)
 {
     return plan_bp(P, values, nullptr, nullptr, 0, 0.0f, mode, accumulate, image, (hipStream_t)stream);
+}
+
+int srf_plan_backproject_update(srf_plan *P, const float *values, float *x, const float *emap, float zero_guard,
+                                void *stream)
+{
+    if (!P || !x || !emap) return SRF_ERR_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (P->nitems > 0)
+        PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values,
+                                                                       nullptr, nullptr, 0, 0.0f, 0, P->G, P->T,
+                                                                       P->rec_ev.p, P->tile_buf.p));
+    dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
+    hipLaunchKernelGGL(k_tile_update, grid, dim3(256), 0, s, P->tile_buf.p, P->tile_items.p, P->G, P->T, emap,
+                       zero_guard, x);
+    return launch_status();
 }
 
 int srf_plan_backproject_ratio(srf_plan *P, const float *y, const float *p, srf_ratio_kind_t kind, float eps,

@@ -172,7 +172,8 @@ class EmEngine:
         y = self.y if y is None else y
         if plan is not None:
             q = plan.project_ratio(self.x, y, self.ratio_kind, self.eps)
-            bp = plan.backproject(q, 0)
+            plan.backproject_update(q, self.x, self.emap_k, self.zero_guard)   # tile sums fused with the update
+            return
         else:
             lors = self.lors if lors is None else lors
             p = ops.project(lors, self.x, self.shape, self.geom, self.use_tof, self.tof_bin, self.tof_sigma2)

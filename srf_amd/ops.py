@@ -313,6 +313,19 @@ class Plan:
     def backproject(self, values: Optional[Tensor] = None, mode: int = 0) -> Tensor:
         return plan_backproject(self.handle, values, self.n, self.shape, int(mode))
 
+    def backproject_update(self, values: Tensor, x_k: Tensor, emap_k: Tensor, zero_guard: float = 0.0):
+        """x_k <- x_k * backproject(values) / emap_k in place (srf_plan_backproject_update)."""
+        _req(values, "values")
+        if values.numel() != self.n:
+            raise ValueError("values must have one entry per event")
+        _check_image(x_k, self.shape)
+        _check_image(emap_k, self.shape)
+        with torch.cuda.device(x_k.device):
+            rc = lib().srf_plan_backproject_update(self.handle, values.data_ptr(), x_k.data_ptr(), emap_k.data_ptr(),
+                                                   float(zero_guard), stream_handle(x_k.device))
+        check(rc, "srf_plan_backproject_update")
+        return x_k
+
     def backproject_ratio(self, y: Tensor, p: Tensor, kind: int, eps: float, mode: int = 0) -> Tensor:
         return plan_backproject_ratio(self.handle, y, p, int(kind), float(eps), self.n, self.shape, int(mode))
 

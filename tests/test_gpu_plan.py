@@ -213,3 +213,24 @@ def test_plan_rebuild_equals_fresh_plan(tof):
             assert float(plan.backproject(None, 0).abs().sum()) == 0.0
         fresh.close()
     plan.close()
+
+
+def test_backproject_update_equals_two_calls():
+    """srf_plan_backproject_update == srf_plan_backproject then srf_em_update, bit for bit
+    (emap zeros guarded)."""
+    import torch
+    from srf_amd import ops
+    from srf_amd.functions import uniform_cylinder_lors
+    shape, geom = (64, 64, 32), [0.0, 0.0, 0.0, 256.0, 256.0, 128.0]
+    L = uniform_cylinder_lors(50000, 140.0, 120.0, 9, torch.device("cuda"))
+    plan = ops.Plan(L, shape, geom, False)
+    g = torch.Generator("cuda").manual_seed(5)
+    q = torch.rand(50000, device="cuda", generator=g) + 0.5
+    x = torch.rand((32, 64, 64), device="cuda", generator=g) + 0.1
+    emap = torch.rand((32, 64, 64), device="cuda", generator=g)
+    emap[::3, ::5, ::7] = 0.0
+    ref = x.clone()
+    ops.em_update(ref, plan.backproject(q, 0), emap, 1e8)
+    fused = plan.backproject_update(q, x.clone(), emap, 1e8)
+    assert torch.equal(fused, ref)
+    plan.close()
