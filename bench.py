@@ -149,8 +149,11 @@ def main():
     ev = []
 
     def sub_iter(lors_s, y_s, plan, timed):
+        # timed: events around each half; on the plan path also around the tiled kernels alone
         if timed:
-            a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            a, b, c, fa, fb, ba, bb = (torch.cuda.Event(enable_timing=True) for _ in range(7))
+            if plan is not None:
+                plan.time_next(fa, fb)
             a.record(stream)
         if plan is not None:
             q = plan.project_ratio(eng.x, y_s, 1, 1e-5)
@@ -158,6 +161,8 @@ def main():
             p = ops.project(lors_s, eng.x, eng.shape, eng.geom, True, tof_bin, tof_sigma2)
         if timed:
             b.record(stream)
+            if plan is not None:
+                plan.time_next(ba, bb)
         if plan is not None:
             # backprojection with its tile sums fused into the EM update
             plan.backproject_update(q, eng.x, eng.emap_k, 1e8)
@@ -165,7 +170,7 @@ def main():
             bp = ops.backproject_ratio(lors_s, y_s, p, 1, 1e-5, eng.shape, eng.geom, True, tof_bin, tof_sigma2, 0)
         if timed:
             c.record(stream)
-            ev.append((a, b, c))
+            ev.append((a, b, c, (fa, fb, ba, bb) if plan is not None else None))
         if plan is None:
             ops.em_update(eng.x, bp, eng.emap_k, 1e8)
 
@@ -187,8 +192,15 @@ def main():
     n_used = eng.n_per_sub * eng.n_sub
     n_sub = eng.n_sub
     value = n_used * args.steps / t
-    fp_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
-    bp_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+    # halves: projection + per-event reduction, backprojection + tile sums / EM update
+    fp_half_ms = float(np.mean([a.elapsed_time(b) for a, b, _, _ in ev]))
+    bp_half_ms = float(np.mean([b.elapsed_time(c) for _, b, c, _ in ev]))
+    if ev and ev[0][3] is not None:
+        # the tiled kernels alone, from events the library records around their launches
+        fp_ms = float(np.mean([k[0].elapsed_time(k[1]) for _, _, _, k in ev]))
+        bp_ms = float(np.mean([k[2].elapsed_time(k[3]) for _, _, _, k in ev]))
+    else:
+        fp_ms, bp_ms = fp_half_ms, bp_half_ms
     nvox = shape[0] * shape[1] * shape[2]
     # algorithmic bytes per launch: LOR (24) + tof (4) + event values + 4 per term of the reference's
     # algorithm (T_e of the reference term set, as the oracle counts it), + the image once.  Event
@@ -213,7 +225,12 @@ def main():
     roof = {"bound": "hbm", "achieved": dom[2] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": dom[2] / HBM_PEAK, "traffic": traffic, "kernel": dom[0], "mean_launch_ms": dom[1],
             "algorithmic_bytes_per_launch": dom[3],
-            "other": {"project_ms": fp_ms, "backproject_ms": bp_ms, "project_GBps": fp_bw / 1e9,
+            "other": {"project_ms": fp_ms, "backproject_ms": bp_ms,
+                      # whole halves: + per-event reduction / + tile sums and EM update
+                      "project_half_ms": fp_half_ms, "backproject_half_ms": bp_half_ms,
+                      "frac_over_halves": {"project": fp_bytes / (fp_half_ms * 1e-3) / HBM_PEAK,
+                                           "backproject": bp_bytes / (bp_half_ms * 1e-3) / HBM_PEAK},
+                      "project_GBps": fp_bw / 1e9,
                       "backproject_GBps": bp_bw / 1e9, "terms_per_event_fp": te_fp, "terms_per_event_bp": te_bp,
                       "terms_per_event_reference_fp": te_ref_fp, "terms_per_event_reference_bp": te_ref_bp,
                       "frac_over_terms_computed": {

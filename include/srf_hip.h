@@ -208,12 +208,12 @@ int srf_plan_project_ratio(srf_plan *plan, const float *image, const float *y, s
 /* image (+)= backprojection (srf_backproject semantics; values NULL = ones). */
 int srf_plan_backproject(srf_plan *plan, const float *values, srf_bp_mode_t mode,
                          int accumulate, float *image, void *stream);
-/* image (+)= backprojection of ratio(y, p) (srf_backproject_ratio semantics). */
 /* One EM update from a plan: x = x * srf_plan_backproject(values, reference
  * mode) / emap (emap 0 -> zero_guard when it is not 0), the tile sums fused
  * with srf_em_update's operations (bit-identical to the two calls). */
 int srf_plan_backproject_update(srf_plan *plan, const float *values, float *x, const float *emap,
                                 float zero_guard, void *stream);
+/* image (+)= backprojection of ratio(y, p) (srf_backproject_ratio semantics). */
 int srf_plan_backproject_ratio(srf_plan *plan, const float *y, const float *p,
                                srf_ratio_kind_t kind, float eps, srf_bp_mode_t mode,
                                int accumulate, float *image, void *stream);
@@ -221,6 +221,12 @@ int srf_plan_backproject_ratio(srf_plan *plan, const float *y, const float *p,
  * proves that every term lands in exactly one tile segment. */
 int srf_plan_digest(srf_plan *plan, srf_bp_mode_t mode, uint64_t *hash, int32_t *count,
                     void *stream);
+/* Measurement hook, no counterpart in the reference: the next projection or
+ * backprojection of this plan records the two hipEvent_t handles on its stream
+ * immediately before and after its tiled kernel (not the per-event reduction
+ * or tile sums around it), so a caller can time that kernel alone.  One-shot:
+ * the events are dropped after that launch. */
+int srf_plan_time_next(srf_plan *plan, void *before_event, void *after_event);
 
 /* -------------------------------------------------------------- batches --
  * Many reconstructions (scans of different sizes and event counts) as one

@@ -114,6 +114,7 @@ SIGNATURES = [
     ("srf_plan_backproject_ratio", ctypes.c_int, [_P, _P, _P, ctypes.c_int, _F, ctypes.c_int, ctypes.c_int, _P,
                                                   _P]),
     ("srf_plan_digest", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P]),
+    ("srf_plan_time_next", ctypes.c_int, [_P, _P, _P]),
     ("srf_plan_project_ratio", ctypes.c_int, [_P, _P, _P, ctypes.c_int, _F, _P, _P]),
     ("srf_sss_rows_bytes", ctypes.c_size_t, [_I32, _I32]),
     ("srf_sss_prepare", ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _P, ctypes.c_size_t, _P]),

@@ -911,6 +911,7 @@ __global__ __launch_bounds__(256) void k_item_wmax(const SegRec *__restrict__ re
 // The 256 events of a block own one contiguous range of partials: it is
 // staged in LDS with coalesced loads (when it fits), then every thread sums
 // its event's partials in order -- the same additions as a direct loop.
+constexpr int SEG_LOADS = 8;      // staging loads in flight per thread
 constexpr int SEG_STAGE = 6144;   // partials per block staged (24 KiB; ~2300 at 9 segments per event)
 
 __global__ __launch_bounds__(256) void k_seg_reduce(const float *__restrict__ partial, const int *__restrict__ ev_off,
@@ -925,7 +926,21 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const float *__restrict__ pa
     const int s0 = ev_off[e0], s1 = ev_off[eb];
     const bool staged = s1 - s0 <= SEG_STAGE;
     if (staged) {
-        for (int k = s0 + (int)threadIdx.x; k < s1; k += 256) st[k - s0] = partial[k];
+        // SEG_LOADS loads per thread issued before their LDS stores, so they
+        // are in flight together (a load-store loop waits for each load)
+        for (int k0 = s0; k0 < s1; k0 += SEG_LOADS * 256) {
+            float v[SEG_LOADS];
+#pragma unroll
+            for (int j = 0; j < SEG_LOADS; ++j) {
+                const int k = k0 + j * 256 + (int)threadIdx.x;
+                v[j] = k < s1 ? partial[k] : 0.0f;
+            }
+#pragma unroll
+            for (int j = 0; j < SEG_LOADS; ++j) {
+                const int k = k0 + j * 256 + (int)threadIdx.x;
+                if (k < s1) st[k - s0] = v[j];
+            }
+        }
         __syncthreads();
     }
     if (e >= n) return;
@@ -1129,6 +1144,8 @@ struct srf_plan {
     DevBuf<Item> items;
     DevBuf<int> tile_items;
     DevBuf<int> order;   // launch order of the items: estimated work, largest first
+    // one-shot timing events around the next tiled kernel (srf_plan_time_next)
+    hipEvent_t ev_before = nullptr, ev_after = nullptr;
     DevBuf<int> ev_off;
     DevBuf<uint8_t> ev_flag;
     DevBuf<float> partial;
@@ -1490,15 +1507,37 @@ int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments, 
         }                                                                                            \
     } while (0)
 
+// Records the pending timing events (if any) before / after the tiled kernel;
+// they are consumed by that one launch.
+static void mark_before(srf_plan *P, hipStream_t s)
+{
+    if (P->ev_before) (void)hipEventRecord(P->ev_before, s);
+}
+static void mark_after(srf_plan *P, hipStream_t s)
+{
+    if (P->ev_after) (void)hipEventRecord(P->ev_after, s);
+    P->ev_before = P->ev_after = nullptr;
+}
+
+int srf_plan_time_next(srf_plan *P, void *before, void *after)
+{
+    if (!P || !before || !after) return SRF_ERR_INVALID_ARG;
+    P->ev_before = (hipEvent_t)before;
+    P->ev_after = (hipEvent_t)after;
+    return SRF_OK;
+}
+
 int srf_plan_project(srf_plan *P, const float *image, float *out, void *stream)
 {
     if (!P) return SRF_ERR_INVALID_ARG;
     if (P->n == 0) return SRF_OK;
     if (!image || !out) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
+    mark_before(P, s);
     if (P->nitems > 0)
         PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, image, P->G, P->T,
                                                                        P->partial.p));
+    mark_after(P, s);
     hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial.p, P->ev_off.p,
                        P->ev_flag.p, P->n, (const float *)nullptr, 0, 0.0f, out);
     return launch_status();
@@ -1512,9 +1551,11 @@ int srf_plan_project_ratio(srf_plan *P, const float *image, const float *y, srf_
     if (P->n == 0) return SRF_OK;
     if (!image || !y || !q) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
+    mark_before(P, s);
     if (P->nitems > 0)
         PLAN_DISPATCH(k_fp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, image, P->G, P->T,
                                                                        P->partial.p));
+    mark_after(P, s);
     hipLaunchKernelGGL(k_seg_reduce, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, P->partial.p, P->ev_off.p,
                        P->ev_flag.p, P->n, y, (int)kind, eps, q);
     return launch_status();
@@ -1525,10 +1566,12 @@ static int plan_bp(srf_plan *P, const float *values, const float *y, const float
 {
     if (!P || !image) return SRF_ERR_INVALID_ARG;
     if (mode != SRF_BP_REFERENCE && mode != SRF_BP_ADJOINT) return SRF_ERR_INVALID_ARG;
+    mark_before(P, s);
     if (P->nitems > 0)
         PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values, y, p, kind, eps,
                                                                        (int)(mode == SRF_BP_ADJOINT), P->G, P->T,
                                                                        P->rec_ev.p, P->tile_buf.p));
+    mark_after(P, s);
     dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
     hipLaunchKernelGGL(k_tile_reduce, grid, dim3(256), 0, s, P->tile_buf.p, P->tile_items.p, P->G, P->T, accumulate,
                        image);
@@ -1564,10 +1607,12 @@ int srf_plan_backproject_update(srf_plan *P, const float *values, float *x, cons
 {
     if (!P || !x || !emap) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
+    mark_before(P, s);
     if (P->nitems > 0)
         PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values,
                                                                        nullptr, nullptr, 0, 0.0f, 0, P->G, P->T,
                                                                        P->rec_ev.p, P->tile_buf.p));
+    mark_after(P, s);
     dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
     hipLaunchKernelGGL(k_tile_update, grid, dim3(256), 0, s, P->tile_buf.p, P->tile_items.p, P->G, P->T, emap,
                        zero_guard, x);

@@ -329,6 +329,15 @@ class Plan:
     def backproject_ratio(self, y: Tensor, p: Tensor, kind: int, eps: float, mode: int = 0) -> Tensor:
         return plan_backproject_ratio(self.handle, y, p, int(kind), float(eps), self.n, self.shape, int(mode))
 
+    def time_next(self, before, after):
+        """Record torch.cuda.Event before / after the tiled kernel of the next
+        projection or backprojection of this plan (srf_plan_time_next).  Both
+        events are recorded once here first, so that torch has created them."""
+        stream = torch.cuda.current_stream(self.device)
+        before.record(stream)
+        after.record(stream)
+        check(lib().srf_plan_time_next(self.handle, before.cuda_event, after.cuda_event), "srf_plan_time_next")
+
     def digest(self, mode: int):
         h = torch.empty(self.n, dtype=torch.int64, device=self.device)
         c = torch.empty(self.n, dtype=torch.int32, device=self.device)

@@ -234,3 +234,28 @@ def test_backproject_update_equals_two_calls():
     fused = plan.backproject_update(q, x.clone(), emap, 1e8)
     assert torch.equal(fused, ref)
     plan.close()
+
+
+def test_time_next_brackets_the_tiled_kernel():
+    """srf_plan_time_next: the events bracket one launch (a positive interval no
+    longer than the whole call), are dropped after it, and change no result."""
+    import torch
+    from srf_amd import ops
+    from srf_amd.functions import uniform_cylinder_lors
+    shape, geom = (64, 64, 32), [0.0, 0.0, 0.0, 256.0, 256.0, 128.0]
+    L = uniform_cylinder_lors(200000, 140.0, 120.0, 11, torch.device("cuda"))
+    plan = ops.Plan(L, shape, geom, False)
+    x = torch.rand((32, 64, 64), device="cuda", generator=torch.Generator("cuda").manual_seed(3)) + 0.1
+    ref = plan.project(x)
+    a, b, c, d = (torch.cuda.Event(enable_timing=True) for _ in range(4))
+    plan.time_next(a, b)
+    c.record()
+    out = plan.project(x)
+    d.record()
+    again = plan.project(x)   # no pending events: nothing recorded
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref) and torch.equal(again, ref)
+    k, whole = a.elapsed_time(b), c.elapsed_time(d)
+    print(f"kernel {k:.4f} ms of {whole:.4f} ms", flush=True)
+    assert 0.0 < k <= whole
+    plan.close()
