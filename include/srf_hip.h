@@ -285,7 +285,10 @@ int srf_batch_em_step(srf_batch *batch, srf_ratio_kind_t kind, float eps, float 
  *   rows    workspace of srf_sss_rows_bytes() bytes (16-byte aligned)
  *   bins    (n_bins, 2)  detector pairs; out (n_bins)
  *   tables  543 floats: 511/E', window efficiency, KN/sigma_KN per integer
- *           scattering angle 0..180 degrees. */
+ *           scattering angle 0..180 degrees.
+ * Below 32768 bins (a few rings, e.g. a surrogate's check subset) each bin's
+ * sum over points is split across a wavefront and combined in a fixed order;
+ * above, one lane sums one bin serially.  The two agree to float rounding. */
 size_t srf_sss_rows_bytes(int32_t n_det, int32_t n_points);
 int srf_sss_prepare(const float *det, int32_t n_det, const float *points, int32_t n_points,
                     const float *em_int, const float *mu_int, void *rows, size_t rows_bytes, void *stream);

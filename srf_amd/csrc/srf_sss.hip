@@ -13,6 +13,7 @@
 //   k_sss_bins    per sinogram bin: the sum over scatter points, one bin per
 //                 lane.  Bins are ordered by ring and detector A, so the lanes
 //                 of a wavefront share A's row (a broadcast read) and stream B's.
+//   k_sss_bins_wave  the same with one wavefront per bin (small sinograms).
 // Energy-dependent factors depend only on the integer scattering angle
 // (the reference quantises it to whole degrees, kn_formula.py:41), so E', the
 // window efficiency and the KN factor are 181-entry tables.
@@ -89,6 +90,54 @@ __global__ __launch_bounds__(256) void k_sss_bins(const PairRow *__restrict__ ro
     }
     out[b] = acc * scale;
 }
+
+// The same sum with one wavefront per bin, for sinograms too small to fill
+// the GPU one bin per lane (a few rings: a check subset).  Lane j takes the
+// points s = j, j + 64, ...; the A and B rows are then read contiguously.
+// The 64 partial sums are combined by a fixed butterfly, so the result is
+// deterministic (it differs from the serial order of k_sss_bins only by
+// float rounding).
+constexpr int SSS_WAVE = 64;
+
+__global__ __launch_bounds__(256) void k_sss_bins_wave(const PairRow *__restrict__ rows,
+                                                       const float *__restrict__ sp, int nd, int ns,
+                                                       const int *__restrict__ bins, int nb,
+                                                       const float *__restrict__ tables, float scale,
+                                                       float *__restrict__ out)
+{
+    __shared__ float tab[543];
+    for (int i = threadIdx.x; i < 543; i += 256) tab[i] = tables[i];
+    __syncthreads();
+    const int b = blockIdx.x * (256 / SSS_WAVE) + (int)(threadIdx.x / SSS_WAVE);
+    const int lane = (int)(threadIdx.x % SSS_WAVE);
+    if (b >= nb) return;   // whole wavefronts leave together
+    const int da = bins[2 * b], db = bins[2 * b + 1];
+    if (da < 0 || da >= nd || db < 0 || db >= nd) {
+        if (lane == 0) out[b] = __builtin_nanf("");
+        return;
+    }
+    const PairRow *ra = rows + (int64_t)da * ns;
+    const PairRow *rb = rows + (int64_t)db * ns;
+    float acc = 0.0f;
+    for (int s = lane; s < ns; s += SSS_WAVE) {
+        const float m = sp[4 * s + 3];
+        if (m == 0.0f) continue;
+        const PairRow A = ra[s], B = rb[s];
+        const float x = -(A.ux * B.ux + A.uy * B.uy + A.uz * B.uz);
+        int k = (int)(acosf(fminf(1.0f, fmaxf(-1.0f, x))) * 57.295779513082323f);
+        k = min(k, 180);
+        const float rr = tab[k];
+        const float Ia = __expf(-(A.mu + B.mu * rr)) * A.em;
+        const float Ib = __expf(-(A.mu * rr + B.mu)) * B.em;
+        acc += A.g * B.g * tab[181 + k] * tab[362 + k] * m * (Ia + Ib);
+    }
+#pragma unroll
+    for (int off = SSS_WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, SSS_WAVE);
+    if (lane == 0) out[b] = acc * scale;
+}
+
+// Sinograms below this many bins use k_sss_bins_wave.
+constexpr int SSS_WAVE_BINS = 32768;
 
 // work[0] += sum_e sino[bins[e]], work[1] += sum_e y[e] (double, one atomic per block)
 __device__ __forceinline__ float bin_value(const float *__restrict__ sino, const int32_t *__restrict__ bins,
@@ -177,8 +226,15 @@ int srf_sss_sinogram(const void *rows, int32_t n_det, const float *points, int32
     if (n_bins < 0 || n_det <= 0 || n_points <= 0) return SRF_ERR_INVALID_ARG;
     if (n_bins == 0) return SRF_OK;
     if (!rows || !points || !bins || !tables || !out) return SRF_ERR_INVALID_ARG;
-    hipLaunchKernelGGL(k_sss_bins, dim3((unsigned)((n_bins + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                       (const PairRow *)rows, points, n_det, n_points, bins, n_bins, tables, scale, out);
+    if (n_bins < SSS_WAVE_BINS) {
+        constexpr int per_block = 256 / SSS_WAVE;
+        hipLaunchKernelGGL(k_sss_bins_wave, dim3((unsigned)((n_bins + per_block - 1) / per_block)), dim3(256), 0,
+                           (hipStream_t)stream, (const PairRow *)rows, points, n_det, n_points, bins, n_bins, tables,
+                           scale, out);
+    } else {
+        hipLaunchKernelGGL(k_sss_bins, dim3((unsigned)((n_bins + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                           (const PairRow *)rows, points, n_det, n_points, bins, n_bins, tables, scale, out);
+    }
     return launch_status();
 }
 

@@ -115,3 +115,20 @@ def test_in_loop_model_on_held_out_osem_iterates():
           f"<=2%: {(errs <= 0.02).mean():.2f}", flush=True)
     assert np.median(errs) <= 0.02
     assert (errs <= 0.02).mean() >= 0.6
+
+
+def test_ring_subset_sss_matches_full(setup):
+    """SSS on a few rings (4096 bins: one wavefront per bin) equals those rings
+    of the full SSS (131072 bins: one lane per bin) to float rounding."""
+    import torch
+    from srf_amd import phantom
+    from srf_amd.sss import SingleScatterSimulation
+    ts, net, meta, g, center, size, sss, feat = setup
+    act, mu, _, _ = phantom.scatter_phantom(ts.SHAPE, ts.VOXEL, ts.HELD_OUT_SEED0 + 4, device="cuda")
+    full = sss(act, mu)
+    rings = (0, 16, 31)
+    sub = SingleScatterSimulation(g, ts.SHAPE, center, size, rings=rings)(act, mu)
+    ref = full[list(rings)]
+    err = float(torch.linalg.vector_norm((sub - ref).double()) / torch.linalg.vector_norm(ref.double()))
+    print(f"rings {rings}: relative L2 {err:.3g} (bins {sub.numel()} vs {full.numel()})", flush=True)
+    assert sub.shape == ref.shape and err <= 1e-5
