@@ -33,6 +33,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "marcher.hpp"
@@ -1175,10 +1176,44 @@ int dmalloc(T **p, size_t count, size_t &total)
     return SRF_OK;
 }
 
-template <class T>
-void dfree(DevBuf<T> &b)
+// Plan memory comes from one stream-ordered pool per device that keeps up to
+// POOL_KEEP bytes of freed memory for reuse.  Many plans built and destroyed
+// in a row (one per OSEM subset of many scans, or chunk after chunk) then
+// reuse pool memory instead of calling hipMalloc / hipFree, which map and
+// unmap memory and wait for the device each time.
+constexpr uint64_t POOL_KEEP = 16ull << 30;
+constexpr int POOL_DEVICES = 64;
+std::mutex pool_mu;
+hipMemPool_t pools[POOL_DEVICES];
+
+hipMemPool_t plan_pool()
 {
-    (void)hipFree(b.p);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= POOL_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(pool_mu);
+    if (!pools[dev]) {
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &props) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        uint64_t keep = POOL_KEEP;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        pools[dev] = pool;
+    }
+    return pools[dev];
+}
+
+// Freed in stream order on s: work already queued on s may still read it.
+template <class T>
+void dfree(DevBuf<T> &b, hipStream_t s)
+{
+    if (b.p) (void)hipFreeAsync(b.p, s);
     b.p = nullptr;
     b.cap = 0;
 }
@@ -1186,15 +1221,21 @@ void dfree(DevBuf<T> &b)
 // At least `count` elements (1 if count is 0), reallocated only to grow.  A
 // buffer that grows again (a rebuild with more segments than any before) gets
 // a quarter of headroom, so that a sequence of similar rebuilds reallocates
-// (and synchronises in hipFree) a few times rather than at every new maximum.
+// (and drains the device) a few times rather than at every new maximum.
 template <class T>
-int ensure(DevBuf<T> &b, size_t count)
+int ensure(DevBuf<T> &b, size_t count, hipStream_t s)
 {
     if (count == 0) count = 1;
     if (b.cap >= count) return SRF_OK;
-    if (b.cap > 0) count += count / 4;
-    dfree(b);
-    if (hipMalloc((void **)&b.p, count * sizeof(T)) != hipSuccess) {
+    if (b.cap > 0) {
+        // a rebuilt plan outgrows this buffer: work on other streams may still
+        // read it, so drain the device before it goes back to the pool
+        count += count / 4;
+        (void)hipDeviceSynchronize();
+        dfree(b, s);
+    }
+    hipMemPool_t pool = plan_pool();
+    if (!pool || hipMallocFromPoolAsync((void **)&b.p, count * sizeof(T), pool, s) != hipSuccess) {
         (void)hipGetLastError();
         b.p = nullptr;
         return SRF_ERR_LAUNCH;
@@ -1203,34 +1244,38 @@ int ensure(DevBuf<T> &b, size_t count)
     return SRF_OK;
 }
 
-void free_scratch(srf_plan *P)
+void free_scratch(srf_plan *P, hipStream_t s)
 {
-    dfree(P->ev_cnt);
-    dfree(P->key_count);
-    dfree(P->key_cursor);
-    dfree(P->scan_tmp);
-    dfree(P->sort_tmp);
-    dfree(P->sk);
-    dfree(P->sk2);
-    dfree(P->idx);
-    dfree(P->idx2);
-    dfree(P->rec_ev2);
-    dfree(P->recs2);
+    dfree(P->ev_cnt, s);
+    dfree(P->key_count, s);
+    dfree(P->key_cursor, s);
+    dfree(P->scan_tmp, s);
+    dfree(P->sort_tmp, s);
+    dfree(P->sk, s);
+    dfree(P->sk2, s);
+    dfree(P->idx, s);
+    dfree(P->idx2, s);
+    dfree(P->rec_ev2, s);
+    dfree(P->recs2, s);
 }
 
+// Destroy: the plan may have been used on any stream, so the device is
+// drained first (as hipFree would), then the memory goes back to the pool.
 void plan_free(srf_plan *P)
 {
     if (!P) return;
-    dfree(P->recs);
-    dfree(P->rec_ev);
-    dfree(P->items);
-    dfree(P->tile_items);
-    dfree(P->order);
-    dfree(P->ev_off);
-    dfree(P->ev_flag);
-    dfree(P->partial);
-    dfree(P->tile_buf);
-    free_scratch(P);
+    (void)hipDeviceSynchronize();
+    const hipStream_t s = nullptr;
+    dfree(P->recs, s);
+    dfree(P->rec_ev, s);
+    dfree(P->items, s);
+    dfree(P->tile_items, s);
+    dfree(P->order, s);
+    dfree(P->ev_off, s);
+    dfree(P->ev_flag, s);
+    dfree(P->partial, s);
+    dfree(P->tile_buf, s);
+    free_scratch(P, s);
     delete P;
 }
 
@@ -1313,11 +1358,11 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     P->tof = tof != nullptr;
     P->ds = G.dy != 1.0f;
     P->dz = G.dz != 1.0f;
-    PB_TRY(ensure(P->ev_off, (size_t)n + 1));
-    PB_TRY(ensure(P->ev_flag, (size_t)n));
-    PB_TRY(ensure(P->ev_cnt, (size_t)n + 1));
-    PB_TRY(ensure(P->key_count, (size_t)T.nkeys));
-    PB_TRY(ensure(P->key_cursor, (size_t)T.nkeys));
+    PB_TRY(ensure(P->ev_off, (size_t)n + 1, s));
+    PB_TRY(ensure(P->ev_flag, (size_t)n, s));
+    PB_TRY(ensure(P->ev_cnt, (size_t)n + 1, s));
+    PB_TRY(ensure(P->key_count, (size_t)T.nkeys, s));
+    PB_TRY(ensure(P->key_cursor, (size_t)T.nkeys, s));
     hipLaunchKernelGGL(k_fill_u32, dim3((T.nkeys + 255) / 256), dim3(256), 0, s, P->key_count.p, 0u,
                        (int64_t)T.nkeys);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s,
@@ -1336,7 +1381,7 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     // event offsets: exclusive scan of the per-event counts (ev_off[n] = total)
     size_t tmp_bytes = 0;
     PB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, P->ev_cnt.p, P->ev_off.p, (int)(n + 1), s));
-    PB_TRY(ensure(P->scan_tmp, tmp_bytes));
+    PB_TRY(ensure(P->scan_tmp, tmp_bytes, s));
     PB_HIP(hipcub::DeviceScan::ExclusiveSum(P->scan_tmp.p, tmp_bytes, P->ev_cnt.p, P->ev_off.p, (int)(n + 1), s));
     std::vector<unsigned> kc((size_t)T.nkeys);
     int total = 0;
@@ -1363,13 +1408,13 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     tile_items[T.ntiles] = (int)items.size();
     if ((int64_t)acc != nseg) return SRF_ERR_LAUNCH;
     const int nitems = (int)items.size();
-    PB_TRY(ensure(P->recs, (size_t)nseg));
-    PB_TRY(ensure(P->rec_ev, (size_t)nseg));
-    PB_TRY(ensure(P->partial, (size_t)nseg));
-    PB_TRY(ensure(P->items, items.size()));
-    PB_TRY(ensure(P->tile_items, tile_items.size()));
-    PB_TRY(ensure(P->order, items.size()));
-    PB_TRY(ensure(P->tile_buf, (size_t)std::max(1, nitems) * TVOX));
+    PB_TRY(ensure(P->recs, (size_t)nseg, s));
+    PB_TRY(ensure(P->rec_ev, (size_t)nseg, s));
+    PB_TRY(ensure(P->partial, (size_t)nseg, s));
+    PB_TRY(ensure(P->items, items.size(), s));
+    PB_TRY(ensure(P->tile_items, tile_items.size(), s));
+    PB_TRY(ensure(P->order, items.size(), s));
+    PB_TRY(ensure(P->tile_buf, (size_t)std::max(1, nitems) * TVOX, s));
     if (!items.empty())
         PB_HIP(hipMemcpyAsync(P->items.p, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
     PB_HIP(hipMemcpyAsync(P->tile_items.p, tile_items.data(), tile_items.size() * sizeof(int),
@@ -1386,12 +1431,12 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
         PB_HIP(hipMemcpyAsync(P->order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s));
     PB_HIP(hipMemcpyAsync(P->key_cursor.p, koff.data(), koff.size() * sizeof(unsigned), hipMemcpyHostToDevice, s));
     if (n > 0) {
-        PB_TRY(ensure(P->sk, (size_t)nseg));
-        PB_TRY(ensure(P->sk2, (size_t)nseg));
-        PB_TRY(ensure(P->idx, (size_t)nseg));
-        PB_TRY(ensure(P->idx2, (size_t)nseg));
-        PB_TRY(ensure(P->recs2, (size_t)nseg));
-        PB_TRY(ensure(P->rec_ev2, (size_t)nseg));
+        PB_TRY(ensure(P->sk, (size_t)nseg, s));
+        PB_TRY(ensure(P->sk2, (size_t)nseg, s));
+        PB_TRY(ensure(P->idx, (size_t)nseg, s));
+        PB_TRY(ensure(P->idx2, (size_t)nseg, s));
+        PB_TRY(ensure(P->recs2, (size_t)nseg, s));
+        PB_TRY(ensure(P->rec_ev2, (size_t)nseg, s));
         if (P->tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
                                tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->key_cursor.p, P->recs.p,
@@ -1414,7 +1459,7 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
             size_t sort_bytes = 0;
             PB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, P->sk.p, P->sk2.p, P->idx.p, P->idx2.p,
                                                        (int)nseg, 0, 39 + key_bits, s));
-            PB_TRY(ensure(P->sort_tmp, sort_bytes));
+            PB_TRY(ensure(P->sort_tmp, sort_bytes, s));
             PB_HIP(hipcub::DeviceRadixSort::SortPairs(P->sort_tmp.p, sort_bytes, P->sk.p, P->sk2.p, P->idx.p,
                                                        P->idx2.p, (int)nseg, 0, 39 + key_bits, s));
             hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs.p, P->rec_ev.p, P->idx2.p, nseg,
@@ -1458,7 +1503,7 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride, const f
         plan_free(P);
         return rc;
     }
-    free_scratch(P);
+    free_scratch(P, (hipStream_t)stream);
     *out = P;
     return SRF_OK;
 }
