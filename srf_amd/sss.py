@@ -254,9 +254,29 @@ class SingleScatterSimulation:
         need = int(lib().srf_sss_rows_bytes(self.nd, max(ns, 1))) // 4 + 4
         return torch.empty(need, dtype=torch.float32, device=self.dev)
 
+    MU_CACHE_MAX = 1 << 26   # largest kept set of attenuation integrals (floats)
+
+    def _mu_part(self, mu_xyz):
+        """The scatter points and the attenuation line integrals of mu_xyz.
+        They depend on the mu-map alone, so the last ones are kept and reused
+        while the same tensor comes back unmodified (same object, same
+        in-place version counter; the cache holds a reference, so the tensor
+        cannot be freed and its address reused meanwhile)."""
+        c = getattr(self, "_mu_cache", None)
+        if c is not None and c[0] is mu_xyz and c[1] == mu_xyz._version:
+            return c[2], c[3]
+        pts = self.points(mu_xyz)
+        mu = None
+        if pts.shape[0] > 0:
+            mu = self.line_integrals(mu_xyz, None if not self.compact else pts)
+        # a full-size set (every detector to every point) is not kept
+        keep = mu is None or mu.numel() <= self.MU_CACHE_MAX
+        self._mu_cache = (mu_xyz, mu_xyz._version, pts, mu) if keep else None
+        return pts, mu
+
     def __call__(self, emission_xyz, mu_xyz):
         """(n_rings or len(rings), n_angles, n_radial) float32 scatter sinogram."""
-        pts = self.points(mu_xyz)
+        pts, mu = self._mu_part(mu_xyz)
         out = torch.zeros(self.bins.shape[0], dtype=torch.float32, device=self.dev)
         shape = (self.n_out_rings, self.g.n_angles, self.g.n_radial)
         ns = pts.shape[0]
@@ -264,7 +284,6 @@ class SingleScatterSimulation:
             return out.reshape(shape)
         sel = None if not self.compact else pts
         em = self.line_integrals(emission_xyz, sel)
-        mu = self.line_integrals(mu_xyz, sel)
         rows = self._rows_for(ns)
         sref = stream_handle(self.dev)
         check(lib().srf_sss_prepare(self.det.data_ptr(), self.nd, pts.data_ptr(), ns, em.data_ptr(),

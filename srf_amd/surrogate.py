@@ -327,10 +327,20 @@ class SinogramFeatures:
         img_k = ops.to_kernel_layout(img_xyz.to(self.dev, torch.float32).contiguous())
         return ops.project(self.lors, img_k, self.shape, self.geom, False, 0.0, 0.0) * self.to_integral
 
+    def _mu_integrals(self, mu_xyz):
+        # the mu-map's integrals, kept while the same unmodified tensor comes back
+        # (the cache holds a reference, so its address cannot be reused)
+        c = getattr(self, "_mu_cache", None)
+        if c is not None and c[0] is mu_xyz and c[1] == mu_xyz._version:
+            return c[2]
+        mu = self.integrals(mu_xyz)
+        self._mu_cache = (mu_xyz, mu_xyz._version, mu)
+        return mu
+
     def __call__(self, em_xyz, mu_xyz):
         """-> (x (n, 8) int16 bf16, m (1,) float32 mean emission integral)."""
         em = self.integrals(em_xyz)
-        mu = self.integrals(mu_xyz)
+        mu = self._mu_integrals(mu_xyz)
         m = em.mean().reshape(1).clamp_min(1e-30)
         scale = torch.cat([1.0 / m, COARSE_KAPPA / m]).contiguous()
         extra = None

@@ -63,3 +63,27 @@ def test_scatter_correct_subtracts_fraction():
     out = sc(Image(act.numpy(), center, size), Umap(mu.numpy(), center, size), None, lm)
     assert out.data.shape == (20000,)
     assert float(np.sum(lm.data) - np.sum(out.data)) == pytest.approx(0.3 * 20000, rel=1e-4)
+
+
+def test_mu_integrals_kept_for_the_same_mu_tensor():
+    """The scatter points and attenuation integrals are reused while the same
+    mu tensor comes back unmodified; an in-place change or another tensor
+    recomputes them (results equal to a fresh simulation each time)."""
+    from srf_amd.sss import SingleScatterSimulation
+    g = _small_geometry()
+    act, mu, center, size = _phantom()
+    act, mu = act.cuda(), mu.cuda()
+    sss = SingleScatterSimulation(g, act.shape, center, size)
+    first = sss(act, mu)
+    assert sss._mu_cache is not None and sss._mu_cache[0] is mu
+    kept = sss._mu_cache[3]
+    again = sss(act * 2.0, mu)
+    assert sss._mu_cache[3] is kept     # reused
+    assert torch.equal(again, SingleScatterSimulation(g, act.shape, center, size)(act * 2.0, mu))
+    assert torch.equal(first, SingleScatterSimulation(g, act.shape, center, size)(act, mu))
+    mu.mul_(1.5)                         # in place: the version counter moves
+    changed = sss(act, mu)
+    assert sss._mu_cache[3] is not kept
+    assert torch.equal(changed, SingleScatterSimulation(g, act.shape, center, size)(act, mu))
+    other = mu.clone()
+    assert torch.equal(sss(act, other), changed) and sss._mu_cache[0] is other
