@@ -633,6 +633,7 @@ __device__ __forceinline__ void walk_for(TileWalk &t, float *lds, int dummy, int
 
 // Projector LDS: the halo tile plus one zero word, the target of rejected terms.
 constexpr int LDS_WORDS = HPAD + 1;
+constexpr int STAGE_LOADS = (HX * HY * HZ + TILE_THREADS - 1) / TILE_THREADS;   // 12 per thread
 
 // Projection of one work item: stage its tile in LDS, march every record,
 // one partial sum per record.
@@ -643,13 +644,24 @@ __device__ __forceinline__ void fp_item(const SegRec *__restrict__ recs, const I
 {
     int x0, y0, z0;
     tile_origin(it.tile, T, x0, y0, z0);
-    for (int i = threadIdx.x; i < HX * HY * HZ; i += TILE_THREADS) {
+    // every load of the staging is issued before the first LDS store (a
+    // load-store loop would wait for each load in turn)
+    float v[STAGE_LOADS];
+#pragma unroll
+    for (int j = 0; j < STAGE_LOADS; ++j) {
+        const int i = (int)threadIdx.x + j * TILE_THREADS;
         const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
         const int lx = hx - 1, ly = hy - 1, lz = hz - 1;
         const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
         const bool core = (unsigned)lx < (unsigned)TX && (unsigned)ly < (unsigned)TY && (unsigned)lz < (unsigned)TZ;
-        tile[hx + HSY * hy + HSZ * hz] =
-            (core && gx < G.nx && gy < G.ny && gz < G.nz) ? img[gx + G.nx * (gy + G.ny * gz)] : 0.0f;
+        v[j] = (i < HX * HY * HZ && core && gx < G.nx && gy < G.ny && gz < G.nz) ? img[gx + G.nx * (gy + G.ny * gz)]
+                                                                               : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < STAGE_LOADS; ++j) {
+        const int i = (int)threadIdx.x + j * TILE_THREADS;
+        const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+        if (i < HX * HY * HZ) tile[hx + HSY * hy + HSZ * hz] = v[j];
     }
     if (threadIdx.x == 0) tile[HPAD] = 0.0f;
     __syncthreads();
