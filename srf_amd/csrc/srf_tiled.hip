@@ -841,17 +841,25 @@ __global__ __launch_bounds__(TILE_THREADS) void k_digest_tiled(const SegRec *__r
 // loads issued four at a time).
 __device__ __forceinline__ float sum_items(const float *__restrict__ buf, int k0, int k1, int i)
 {
+    // up to 8 loads in flight (a tile has ~8 items at the usual sizes); the
+    // additions stay in item order
     float s = 0.0f;
     int k = k0;
-    for (; k + 4 <= k1; k += 4) {
-        const float a0 = buf[(int64_t)k * TVOX + i], a1 = buf[(int64_t)(k + 1) * TVOX + i];
-        const float a2 = buf[(int64_t)(k + 2) * TVOX + i], a3 = buf[(int64_t)(k + 3) * TVOX + i];
-        s += a0;
-        s += a1;
-        s += a2;
-        s += a3;
+    for (; k + 8 <= k1; k += 8) {
+        float a[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = buf[(int64_t)(k + j) * TVOX + i];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s += a[j];
     }
-    for (; k < k1; ++k) s += buf[(int64_t)k * TVOX + i];
+    if (k < k1) {
+        float a[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = k + j < k1 ? buf[(int64_t)(k + j) * TVOX + i] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (k + j < k1) s += a[j];
+    }
     return s;
 }
 
@@ -936,6 +944,11 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const float *__restrict__ pa
     const int64_t e0 = (int64_t)blockIdx.x * 256;
     const int64_t e = e0 + threadIdx.x;
     const int64_t eb = min(e0 + 256, n);
+    // the event's own words are loaded first, in flight with the staging
+    const bool live = e < n;
+    const int k_lo = live ? ev_off[e] : 0, k_hi = live ? ev_off[e + 1] : 0;
+    const bool flag = live && ev_flag[e];
+    const float ye = (live && y) ? y[e] : 0.0f;
     const int s0 = ev_off[e0], s1 = ev_off[eb];
     const bool staged = s1 - s0 <= SEG_STAGE;
     if (staged) {
@@ -956,17 +969,17 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const float *__restrict__ pa
         }
         __syncthreads();
     }
-    if (e >= n) return;
+    if (!live) return;
     float s = 0.0f;
     if (staged) {
-        for (int k = ev_off[e]; k < ev_off[e + 1]; ++k) s += st[k - s0];
+        for (int k = k_lo; k < k_hi; ++k) s += st[k - s0];
     } else {
-        for (int k = ev_off[e]; k < ev_off[e + 1]; ++k) s += partial[k];
+        for (int k = k_lo; k < k_hi; ++k) s += partial[k];
     }
-    if (ev_flag[e]) s = __builtin_nanf("");
+    if (flag) s = __builtin_nanf("");
     if (y) {
         const float d = s + eps;
-        float q = y[e] / d;
+        float q = ye / d;
         if (kind == SRF_RATIO_DIV_NO_NAN && d == 0.0f) q = 0.0f;
         s = q;
     }
@@ -1144,6 +1157,7 @@ template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t cap = 0;
+    bool pooled = false;   // from the plan pool (else hipMalloc)
 };
 
 struct srf_plan {
@@ -1197,13 +1211,14 @@ constexpr uint64_t POOL_KEEP = 16ull << 30;
 constexpr int POOL_DEVICES = 64;
 std::mutex pool_mu;
 hipMemPool_t pools[POOL_DEVICES];
+bool pool_failed[POOL_DEVICES];
 
 hipMemPool_t plan_pool()
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= POOL_DEVICES) return nullptr;
     std::lock_guard<std::mutex> lock(pool_mu);
-    if (!pools[dev]) {
+    if (!pools[dev] && !pool_failed[dev]) {
         hipMemPoolProps props = {};
         props.allocType = hipMemAllocationTypePinned;
         props.handleTypes = hipMemHandleTypeNone;
@@ -1212,6 +1227,7 @@ hipMemPool_t plan_pool()
         hipMemPool_t pool = nullptr;
         if (hipMemPoolCreate(&pool, &props) != hipSuccess) {
             (void)hipGetLastError();
+            pool_failed[dev] = true;
             return nullptr;
         }
         uint64_t keep = POOL_KEEP;
@@ -1225,9 +1241,13 @@ hipMemPool_t plan_pool()
 template <class T>
 void dfree(DevBuf<T> &b, hipStream_t s)
 {
-    if (b.p) (void)hipFreeAsync(b.p, s);
+    if (b.p) {
+        if (b.pooled) (void)hipFreeAsync(b.p, s);
+        else (void)hipFree(b.p);
+    }
     b.p = nullptr;
     b.cap = 0;
+    b.pooled = false;
 }
 
 // At least `count` elements (1 if count is 0), reallocated only to grow.  A
@@ -1246,13 +1266,17 @@ int ensure(DevBuf<T> &b, size_t count, hipStream_t s)
         (void)hipDeviceSynchronize();
         dfree(b, s);
     }
+    // a device without memory pools falls back to hipMalloc
     hipMemPool_t pool = plan_pool();
-    if (!pool || hipMallocFromPoolAsync((void **)&b.p, count * sizeof(T), pool, s) != hipSuccess) {
+    const hipError_t e = pool ? hipMallocFromPoolAsync((void **)&b.p, count * sizeof(T), pool, s)
+                              : hipMalloc((void **)&b.p, count * sizeof(T));
+    if (e != hipSuccess) {
         (void)hipGetLastError();
         b.p = nullptr;
         return SRF_ERR_LAUNCH;
     }
     b.cap = count;
+    b.pooled = pool != nullptr;
     return SRF_OK;
 }
 
