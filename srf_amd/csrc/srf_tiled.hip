@@ -267,63 +267,49 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_count(const float *__restri
         if (hist[i]) atomicAdd(&key_count[i], hist[i]);
 }
 
+// Each record is written at its segment slot (ev_off[e] + k: event order, a
+// deterministic position) with its sort key; the sort that follows groups the
+// records by key.
 template <bool TOF>
 __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restrict__ lors, int64_t n, int stride,
                                                           const float *__restrict__ tofv, int tof_stride,
                                                           LaunchGeom G, TileCfg T, float twin,
                                                           const int *__restrict__ ev_off,
-                                                          unsigned *__restrict__ key_cursor,
                                                           SegRec *__restrict__ recs, int *__restrict__ rec_ev,
-                                                          unsigned long long *__restrict__ rec_sk)
+                                                          unsigned *__restrict__ rec_sk)
 {
-    extern __shared__ unsigned hist[];
-    for (int i = threadIdx.x; i < T.nkeys; i += BIN_THREADS) hist[i] = 0;
-    __syncthreads();
     const int64_t e = (int64_t)blockIdx.x * BIN_THREADS + threadIdx.x;
+    if (e >= n) return;
     RaySetup r;
-    const bool act = e < n;
-    if (act) {
-        ray_setup<TOF>(lors + e * stride, TOF ? tofv[e * tof_stride] : 0.0f, G, r);
-        for_each_event_segment<TOF>(r, G, T, twin, [&](int tile, int cls, int, int, const Branch &, const SegCtx &) {
-            atomicAdd(&hist[tile * NCLASS + cls], 1u);
-        });
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < T.nkeys; i += BIN_THREADS)
-        if (hist[i]) hist[i] = atomicAdd(&key_cursor[i], hist[i]);
-    __syncthreads();
-    if (act) {
-        int j = ev_off[e];
-        for_each_event_segment<TOF>(r, G, T, twin, [&](int tile, int cls, int a, int b, const Branch &B,
-                                                       const SegCtx &c) {
-            const unsigned pos = atomicAdd(&hist[tile * NCLASS + cls], 1u);
-            SegRec R;
-            R.p1 = B.p1;
-            R.sec1 = B.sec1;
-            R.ks = B.ks;
-            R.z1 = r.z1;
-            R.kz = B.kz;
-            R.Wbp = branch_weight<TOF>(B, r, G, false);
-            R.Wfp = branch_weight<TOF>(B, r, G, true);
-            tof_coeffs<TOF>(B, r, G, R.tA, R.tB);
-            R.ev = (int)e;
-            R.slot = j++;
-            R.ab_bp = c.bp_use ? pack_ab(a, b) : 0u;
-            const int fa = max(a, c.fa), fb = min(b, c.fb);
-            R.ab_fp = (c.fp_use && fa < fb) ? pack_ab(fa, fb) : 0u;
-            R.meta = B.xdom ? 1u : 0u;
-            R.pad[0] = R.pad[1] = 0u;
-            recs[pos] = R;
-            rec_ev[pos] = (int)e;
-            // sort key: (tile, class), then the branch (a wavefront mostly runs
-            // one branch's marcher), then slab count (longest first: its lanes
-            // march similar lengths), then the segment slot (unique)
-            const unsigned len = (unsigned)min(b - a, 63);
-            rec_sk[pos] = ((unsigned long long)(tile * NCLASS + cls) << 39) |
-                          ((unsigned long long)R.meta << 38) | ((unsigned long long)(63u - len) << 32) |
-                          (unsigned)R.slot;
-        });
-    }
+    ray_setup<TOF>(lors + e * stride, TOF ? tofv[e * tof_stride] : 0.0f, G, r);
+    int j = ev_off[e];
+    for_each_event_segment<TOF>(r, G, T, twin, [&](int tile, int cls, int a, int b, const Branch &B,
+                                                   const SegCtx &c) {
+        const int pos = j++;
+        SegRec R;
+        R.p1 = B.p1;
+        R.sec1 = B.sec1;
+        R.ks = B.ks;
+        R.z1 = r.z1;
+        R.kz = B.kz;
+        R.Wbp = branch_weight<TOF>(B, r, G, false);
+        R.Wfp = branch_weight<TOF>(B, r, G, true);
+        tof_coeffs<TOF>(B, r, G, R.tA, R.tB);
+        R.ev = (int)e;
+        R.slot = pos;
+        R.ab_bp = c.bp_use ? pack_ab(a, b) : 0u;
+        const int fa = max(a, c.fa), fb = min(b, c.fb);
+        R.ab_fp = (c.fp_use && fa < fb) ? pack_ab(fa, fb) : 0u;
+        R.meta = B.xdom ? 1u : 0u;
+        R.pad[0] = R.pad[1] = 0u;
+        recs[pos] = R;
+        rec_ev[pos] = (int)e;
+        // sort key: (tile, class), then the branch (a wavefront mostly runs
+        // one branch's marcher), then slab count (longest first: its lanes
+        // march similar lengths); a stable sort keeps slot order within a key
+        const unsigned len = (unsigned)min(b - a, 63);
+        rec_sk[pos] = ((unsigned)(tile * NCLASS + cls) << 7) | (R.meta << 6) | (63u - len);
+    });
 }
 
 // ------------------------------------------------------------ tile kernels --
@@ -1179,9 +1165,9 @@ struct srf_plan {
     DevBuf<float> tile_buf;
     // binning scratch: freed after srf_plan_create, kept by srf_plan_rebuild
     DevBuf<int> ev_cnt;
-    DevBuf<unsigned> key_count, key_cursor;
+    DevBuf<unsigned> key_count;
     DevBuf<char> scan_tmp, sort_tmp;
-    DevBuf<unsigned long long> sk, sk2;
+    DevBuf<unsigned> sk, sk2;
     DevBuf<int> idx, idx2, rec_ev2;
     DevBuf<SegRec> recs2;
 };
@@ -1284,7 +1270,6 @@ void free_scratch(srf_plan *P, hipStream_t s)
 {
     dfree(P->ev_cnt, s);
     dfree(P->key_count, s);
-    dfree(P->key_cursor, s);
     dfree(P->scan_tmp, s);
     dfree(P->sort_tmp, s);
     dfree(P->sk, s);
@@ -1398,7 +1383,6 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     PB_TRY(ensure(P->ev_flag, (size_t)n, s));
     PB_TRY(ensure(P->ev_cnt, (size_t)n + 1, s));
     PB_TRY(ensure(P->key_count, (size_t)T.nkeys, s));
-    PB_TRY(ensure(P->key_cursor, (size_t)T.nkeys, s));
     hipLaunchKernelGGL(k_fill_u32, dim3((T.nkeys + 255) / 256), dim3(256), 0, s, P->key_count.p, 0u,
                        (int64_t)T.nkeys);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s,
@@ -1425,14 +1409,12 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     PB_HIP(hipMemcpyAsync(&total, P->ev_off.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
     PB_HIP(hipStreamSynchronize(s));
     const int64_t nseg = total;
-    // key offsets and work items (keys are tile-major, so a tile's items are contiguous)
-    std::vector<unsigned> koff((size_t)T.nkeys);
+    // work items from the key counts (keys are tile-major, so a tile's items are contiguous)
     std::vector<Item> items;
     std::vector<int> order;
     std::vector<int> tile_items((size_t)T.ntiles + 1, 0);
     unsigned acc = 0;
     for (int k = 0; k < T.nkeys; ++k) {
-        koff[k] = acc;
         const int tile = k / NCLASS;
         if (k % NCLASS == 0) tile_items[tile] = (int)items.size();
         for (unsigned st = 0; st < kc[k]; st += item_max) {
@@ -1465,7 +1447,6 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     });
     if (!order.empty())
         PB_HIP(hipMemcpyAsync(P->order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    PB_HIP(hipMemcpyAsync(P->key_cursor.p, koff.data(), koff.size() * sizeof(unsigned), hipMemcpyHostToDevice, s));
     if (n > 0) {
         PB_TRY(ensure(P->sk, (size_t)nseg, s));
         PB_TRY(ensure(P->sk2, (size_t)nseg, s));
@@ -1474,30 +1455,31 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
         PB_TRY(ensure(P->recs2, (size_t)nseg, s));
         PB_TRY(ensure(P->rec_ev2, (size_t)nseg, s));
         if (P->tof)
-            hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->key_cursor.p, P->recs.p,
-                               P->rec_ev.p, P->sk.p);
+            hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->rec_ev.p,
+                               P->sk.p);
         else
-            hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->key_cursor.p, P->recs.p,
-                               P->rec_ev.p, P->sk.p);
-        // Deterministic record order: the emit places each record inside its key's
-        // range through atomic cursors, so the order within a key (and with it the
-        // split into work items and each item's fixed-point scale) would vary from
-        // run to run.  Sorting by (key, branch, slab count, segment slot) fixes it
-        // -- every plan of the same events is bit-identical -- and hands each
-        // wavefront one branch's segments of similar length.
+            hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->rec_ev.p,
+                               P->sk.p);
+        // Record order: a stable sort of the slot-ordered records by (key,
+        // branch, slab count) -- within one, slot order.  The order (and with
+        // it the split into work items and each item's fixed-point scale) is a
+        // function of the events alone, so every plan of the same events is
+        // bit-identical, and each wavefront gets one branch's segments of
+        // similar length.
         if (nseg > 0) {
             const unsigned nbs = (unsigned)((nseg + 255) / 256);
             hipLaunchKernelGGL(k_iota_i32, dim3(nbs), dim3(256), 0, s, P->idx.p, nseg);
             int key_bits = 1;
             while ((1 << key_bits) < T.nkeys) ++key_bits;
+            if (7 + key_bits > 32) return SRF_ERR_UNSUPPORTED;
             size_t sort_bytes = 0;
             PB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, P->sk.p, P->sk2.p, P->idx.p, P->idx2.p,
-                                                       (int)nseg, 0, 39 + key_bits, s));
+                                                       (int)nseg, 0, 7 + key_bits, s));
             PB_TRY(ensure(P->sort_tmp, sort_bytes, s));
             PB_HIP(hipcub::DeviceRadixSort::SortPairs(P->sort_tmp.p, sort_bytes, P->sk.p, P->sk2.p, P->idx.p,
-                                                       P->idx2.p, (int)nseg, 0, 39 + key_bits, s));
+                                                       P->idx2.p, (int)nseg, 0, 7 + key_bits, s));
             hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs.p, P->rec_ev.p, P->idx2.p, nseg,
                                P->recs2.p, P->rec_ev2.p);
             PB_HIP(hipGetLastError());
