@@ -275,8 +275,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
                                                           const float *__restrict__ tofv, int tof_stride,
                                                           LaunchGeom G, TileCfg T, float twin,
                                                           const int *__restrict__ ev_off,
-                                                          SegRec *__restrict__ recs, int *__restrict__ rec_ev,
-                                                          unsigned *__restrict__ rec_sk)
+                                                          SegRec *__restrict__ recs, unsigned *__restrict__ rec_sk)
 {
     const int64_t e = (int64_t)blockIdx.x * BIN_THREADS + threadIdx.x;
     if (e >= n) return;
@@ -303,7 +302,6 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
         R.meta = B.xdom ? 1u : 0u;
         R.pad[0] = R.pad[1] = 0u;
         recs[pos] = R;
-        rec_ev[pos] = (int)e;
         // sort key: (tile, class), then the branch (a wavefront mostly runs
         // one branch's marcher), then slab count (longest first: its lanes
         // march similar lengths); a stable sort keeps slot order within a key
@@ -984,16 +982,16 @@ __global__ void k_iota_i32(int *__restrict__ d, int64_t n)
     if (i < n) d[i] = (int)i;
 }
 
-// Records (and their event ids) permuted into the order `idx`.
-__global__ void k_permute_recs(const SegRec *__restrict__ recs, const int *__restrict__ rec_ev,
-                               const int *__restrict__ idx, int64_t n, SegRec *__restrict__ out,
-                               int *__restrict__ out_ev)
+// Records permuted into the order `idx`; their event ids (the records' own
+// ev field) are written beside them for the backprojector's value gathers.
+__global__ void k_permute_recs(const SegRec *__restrict__ recs, const int *__restrict__ idx, int64_t n,
+                               SegRec *__restrict__ out, int *__restrict__ out_ev)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const int j = idx[i];
-    out[i] = recs[j];
-    out_ev[i] = rec_ev[j];
+    const SegRec R = recs[idx[i]];
+    out[i] = R;
+    out_ev[i] = R.ev;
 }
 
 // ---------------------------------------------------------------- batches --
@@ -1168,7 +1166,7 @@ struct srf_plan {
     DevBuf<unsigned> key_count;
     DevBuf<char> scan_tmp, sort_tmp;
     DevBuf<unsigned> sk, sk2;
-    DevBuf<int> idx, idx2, rec_ev2;
+    DevBuf<int> idx, idx2;
     DevBuf<SegRec> recs2;
 };
 
@@ -1276,7 +1274,6 @@ void free_scratch(srf_plan *P, hipStream_t s)
     dfree(P->sk2, s);
     dfree(P->idx, s);
     dfree(P->idx2, s);
-    dfree(P->rec_ev2, s);
     dfree(P->recs2, s);
 }
 
@@ -1453,15 +1450,12 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
         PB_TRY(ensure(P->idx, (size_t)nseg, s));
         PB_TRY(ensure(P->idx2, (size_t)nseg, s));
         PB_TRY(ensure(P->recs2, (size_t)nseg, s));
-        PB_TRY(ensure(P->rec_ev2, (size_t)nseg, s));
         if (P->tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->rec_ev.p,
-                               P->sk.p);
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->sk.p);
         else
             hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->rec_ev.p,
-                               P->sk.p);
+                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->sk.p);
         // Record order: a stable sort of the slot-ordered records by (key,
         // branch, slab count) -- within one, slot order.  The order (and with
         // it the split into work items and each item's fixed-point scale) is a
@@ -1480,11 +1474,10 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
             PB_TRY(ensure(P->sort_tmp, sort_bytes, s));
             PB_HIP(hipcub::DeviceRadixSort::SortPairs(P->sort_tmp.p, sort_bytes, P->sk.p, P->sk2.p, P->idx.p,
                                                        P->idx2.p, (int)nseg, 0, 7 + key_bits, s));
-            hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs.p, P->rec_ev.p, P->idx2.p, nseg,
-                               P->recs2.p, P->rec_ev2.p);
+            hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs.p, P->idx2.p, nseg,
+                               P->recs2.p, P->rec_ev.p);
             PB_HIP(hipGetLastError());
             std::swap(P->recs, P->recs2);
-            std::swap(P->rec_ev, P->rec_ev2);
         }
     }
     if (nitems > 0)
