@@ -76,8 +76,8 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): every box here has one GPU; "
-              "the event-sharded multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs (events sharded "
+              "across GPUs, one plan per share) are not implemented", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -30,8 +30,8 @@ def load_array(path, key=None):
     entry).  No pickled objects are loaded."""
     ext = os.path.splitext(str(path))[1].lower()
     if ext in (".hdf5", ".h5"):
-        raise NotImplementedError(f"{path}: HDF5 object stores need deepdish / h5py, which this build does not "
-                                  "have; save the array as .npy or .npz")
+        raise NotImplementedError(f"{path}: HDF5 object stores need deepdish / h5py, which are not installed; "
+                                  "save the array as .npy or .npz")
     if ext == ".npy":
         return np.load(path, allow_pickle=False)
     if ext == ".npz":

@@ -4,7 +4,7 @@ srf_cnn.hip (BF16 storage, MFMA with FP32 accumulation, Adam on FP32 masters).
 
 The reference has no learned part; this stands in for ScatterCorrect's SSS
 estimate (srfnef/corrections/scattering/scatter.py:25-55) inside the EM loop,
-with plain SSS as the fallback and the judge:
+with plain SSS as the fallback and the reference it is checked against:
 
   features   the emission and attenuation line integrals along every sinogram
              bin's LOR (the projector over the 131072 bin LORs), assembled per
