@@ -886,14 +886,17 @@ __global__ __launch_bounds__(256) void k_tile_update(const float *__restrict__ t
 }
 
 // Per item, the largest weight constant of its records in each mode.
-__global__ __launch_bounds__(256) void k_item_wmax(const SegRec *__restrict__ recs, Item *__restrict__ items)
+// The records' (Wbp, Wfp) pairs are read from the compact copy the
+// permutation writes (8 B per record instead of a 64 B record line).
+__global__ __launch_bounds__(256) void k_item_wmax(const float2 *__restrict__ w, Item *__restrict__ items)
 {
     __shared__ float sb[256], sf[256];
     const Item it = items[blockIdx.x];
     float mb = 0.0f, mf = 0.0f;
     for (int r = it.start + threadIdx.x; r < it.start + it.count; r += 256) {
-        mb = fmaxf(mb, fabsf(recs[r].Wbp));
-        mf = fmaxf(mf, fabsf(recs[r].Wfp));
+        const float2 v = w[r];
+        mb = fmaxf(mb, fabsf(v.x));
+        mf = fmaxf(mf, fabsf(v.y));
     }
     sb[threadIdx.x] = mb;
     sf[threadIdx.x] = mf;
@@ -985,13 +988,14 @@ __global__ void k_iota_i32(int *__restrict__ d, int64_t n)
 // Records permuted into the order `idx`; their event ids (the records' own
 // ev field) are written beside them for the backprojector's value gathers.
 __global__ void k_permute_recs(const SegRec *__restrict__ recs, const int *__restrict__ idx, int64_t n,
-                               SegRec *__restrict__ out, int *__restrict__ out_ev)
+                               SegRec *__restrict__ out, int *__restrict__ out_ev, float2 *__restrict__ out_w)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const SegRec R = recs[idx[i]];
     out[i] = R;
     out_ev[i] = R.ev;
+    out_w[i] = make_float2(R.Wbp, R.Wfp);
 }
 
 // ---------------------------------------------------------------- batches --
@@ -1167,6 +1171,7 @@ struct srf_plan {
     DevBuf<char> scan_tmp, sort_tmp;
     DevBuf<unsigned> sk, sk2;
     DevBuf<int> idx, idx2;
+    DevBuf<float2> wpair;   // (Wbp, Wfp) per record in the sorted order
     DevBuf<SegRec> recs2;
 };
 
@@ -1274,6 +1279,7 @@ void free_scratch(srf_plan *P, hipStream_t s)
     dfree(P->sk2, s);
     dfree(P->idx, s);
     dfree(P->idx2, s);
+    dfree(P->wpair, s);
     dfree(P->recs2, s);
 }
 
@@ -1450,6 +1456,7 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
         PB_TRY(ensure(P->idx, (size_t)nseg, s));
         PB_TRY(ensure(P->idx2, (size_t)nseg, s));
         PB_TRY(ensure(P->recs2, (size_t)nseg, s));
+        PB_TRY(ensure(P->wpair, (size_t)nseg, s));
         if (P->tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
                                tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->sk.p);
@@ -1475,13 +1482,13 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
             PB_HIP(hipcub::DeviceRadixSort::SortPairs(P->sort_tmp.p, sort_bytes, P->sk.p, P->sk2.p, P->idx.p,
                                                        P->idx2.p, (int)nseg, 0, 7 + key_bits, s));
             hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs.p, P->idx2.p, nseg,
-                               P->recs2.p, P->rec_ev.p);
+                               P->recs2.p, P->rec_ev.p, P->wpair.p);
             PB_HIP(hipGetLastError());
             std::swap(P->recs, P->recs2);
         }
     }
     if (nitems > 0)
-        hipLaunchKernelGGL(k_item_wmax, dim3((unsigned)nitems), dim3(256), 0, s, P->recs.p, P->items.p);
+        hipLaunchKernelGGL(k_item_wmax, dim3((unsigned)nitems), dim3(256), 0, s, P->wpair.p, P->items.p);
     PB_HIP(hipGetLastError());
     PB_HIP(hipStreamSynchronize(s));
     P->n = n;
