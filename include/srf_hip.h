@@ -159,7 +159,9 @@ int srf_image_from_kernel_layout(const float *src_kernel, float *dst_xyz, int32_
  * points that allocate or free device memory (the binned records, about 64 B
  * per event-tile segment, plus per-item tile buffers); srf_plan_info reports
  * the total.  The LOR array is read during create / rebuild only, which
- * synchronise `stream`.  The memory comes from a stream-ordered pool per
+ * synchronise `stream`.  An event list may bin into at most 2^31 - 1
+ * segments (SRF_ERR_UNSUPPORTED beyond: split it into several plans, as OSEM
+ * subsets are).  The memory comes from a stream-ordered pool per
  * device that keeps up to 16 GiB of freed plan memory for the next plans
  * (hipMalloc where a device has no pools); srf_plan_destroy first waits for
  * the device, as hipFree does. */

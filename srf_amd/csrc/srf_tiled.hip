@@ -1411,6 +1411,11 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     PB_HIP(hipMemcpyAsync(kc.data(), P->key_count.p, kc.size() * sizeof(unsigned), hipMemcpyDeviceToHost, s));
     PB_HIP(hipMemcpyAsync(&total, P->ev_off.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
     PB_HIP(hipStreamSynchronize(s));
+    // segment slots are 32-bit: a list with more than 2^31 - 1 segments (the
+    // 32-bit prefix sum above would have wrapped) is refused
+    uint64_t kc_sum = 0;
+    for (unsigned c : kc) kc_sum += c;
+    if (kc_sum > (uint64_t)INT_MAX) return SRF_ERR_UNSUPPORTED;
     const int64_t nseg = total;
     // work items from the key counts (keys are tile-major, so a tile's items are contiguous)
     std::vector<Item> items;
