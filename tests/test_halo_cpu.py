@@ -1,0 +1,62 @@
+"""The projector's halo tile (srf_amd/csrc/srf_tiled.hip, march_tile with
+HALO): a cell index clamped into [origin - 1, origin + extent] by one med3
+replaces the per-term tile test (unsigned)(c - origin) < extent.  Checked here
+exhaustively on the CPU over origins, in-image extents and cell indices:
+
+- a cell the test accepts is left unchanged by the clamp;
+- a cell the test rejects is moved to origin - 1 or origin + extent, and that
+  cell is a zero: a halo cell (-1 or TX after subtracting the origin) or a tile
+  cell beyond the image edge, both staged as 0;
+- every clamped index lies inside the halo tile's range [-1, TX], so its LDS
+  word lies inside the tile.
+"""
+import numpy as np
+
+TX, TZ = 32, 8
+
+
+def med3(x, lo, hi):
+    return np.minimum(np.maximum(x, lo), hi)
+
+
+def staged_zero(local, extent, t):
+    """The staging kernel writes 0 at local index -1 and t (the halo) and at
+    local indices >= extent (past the image edge)."""
+    return (local == -1) | (local == t) | (local >= extent)
+
+
+def check_axis(t, n_image):
+    for origin in range(0, n_image, t):
+        extent = min(t, n_image - origin)
+        c = np.arange(-40, n_image + 40)
+        accepted = ((c - origin) >= 0) & ((c - origin) < extent)
+        cl = med3(c, origin - 1, origin + extent)
+        local = cl - origin
+        assert np.array_equal(cl[accepted], c[accepted])
+        assert np.all(staged_zero(local[~accepted], extent, t))
+        assert local.min() >= -1 and local.max() <= t
+
+
+def test_clamp_equals_tile_test_secondary():
+    for n in (32, 33, 50, 64, 96, 200, 256):
+        check_axis(TX, n)
+
+
+def test_clamp_equals_tile_test_axial():
+    for n in (8, 9, 13, 64, 128, 200):
+        check_axis(TZ, n)
+
+
+def test_halo_words_inside_tile():
+    """Word (lx+1) + HSY (ly+1) + HSZ (lz+1) of every cell with -1 <= l <= T
+    lies in [0, HPAD), and the dummy word HPAD is past all of them."""
+    hx, hy, hz = TX + 2, TX + 2, TZ + 2
+    hsy = hx + 1
+    hsz = hsy * hy + 1
+    hpad = hsz * (hz - 1) + hsy * (hy - 1) + hx
+    assert hpad == 11908
+    l = np.arange(-1, TX + 1)
+    lz = np.arange(-1, TZ + 1)
+    words = (l[:, None, None] + 1) + hsy * (l[None, :, None] + 1) + hsz * (lz[None, None, :] + 1)
+    assert words.min() == 0 and words.max() == hpad - 1
+    assert len(np.unique(words)) == words.size     # distinct cells, distinct words
