@@ -1,4 +1,6 @@
-"""The projector's halo tile (srf_amd/csrc/srf_tiled.hip, march_tile with
+"""Integer and bit-level arithmetic of the tiled kernels, checked on the CPU.
+
+The projector's halo tile (srf_amd/csrc/srf_tiled.hip, march_tile with
 HALO): a cell index clamped into [origin - 1, origin + extent] by one med3
 replaces the per-term tile test (unsigned)(c - origin) < extent.  Checked here
 exhaustively on the CPU over origins, in-image extents and cell indices:
@@ -9,6 +11,9 @@ exhaustively on the CPU over origins, in-image extents and cell indices:
   cell beyond the image edge, both staged as 0;
 - every clamped index lies inside the halo tile's range [-1, TX], so its LDS
   word lies inside the tile.
+
+The backprojector's fixed-point terms: one double FMA onto 1.5 * 2^52 gives
+the rounded integer product.
 """
 import numpy as np
 
@@ -60,3 +65,24 @@ def test_halo_words_inside_tile():
     words = (l[:, None, None] + 1) + hsy * (l[None, :, None] + 1) + hsz * (lz[None, None, :] + 1)
     assert words.min() == 0 and words.max() == hpad - 1
     assert len(np.unique(words)) == words.size     # distinct cells, distinct words
+
+
+def test_fixed_point_term_is_rounded_product():
+    """The backprojector's fixed-point term (srf_tiled.hip, fixed_term): the bit
+    pattern of fma(v, r, 1.5 * 2^52) minus 0x4338000000000000 is round(v * r)
+    (ties to even) as a two's-complement int64, for |v * r| < 2^51.  v is a
+    float32 value widened to double and r a float32 fraction, so v * r is exact
+    in double and the fma equals the product plus the constant."""
+    rng = np.random.default_rng(7)
+    v = (rng.standard_normal(200000) * rng.choice([1.0, 1e3, 1e6, 2.0 ** 40], 200000)).astype(np.float32)
+    r = rng.random(200000).astype(np.float32)
+    prod = v.astype(np.float64) * r.astype(np.float64)
+    keep = np.abs(prod) < 2.0 ** 51
+    prod = prod[keep]
+    s = prod + 6755399441055744.0
+    got = s.view(np.int64) - np.int64(0x4338000000000000)
+    assert np.array_equal(got, np.rint(prod).astype(np.int64))
+    # ties go to even, negative values wrap as two's complement
+    t = np.array([2.5, 3.5, -2.5, -0.5, 0.5, -1.0], np.float64)
+    got_t = (t + 6755399441055744.0).view(np.int64) - np.int64(0x4338000000000000)
+    assert got_t.tolist() == [2, 4, -2, 0, 0, -1]
