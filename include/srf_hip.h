@@ -4,8 +4,8 @@
  * Conventions shared by every entry point
  *  - All pointers are device pointers (HBM) unless the comment says otherwise.
  *    The caller owns every buffer; the library allocates device memory only in
- *    srf_plan_create[_ex] and srf_batch_create (freed by the matching
- *    *_destroy).
+ *    srf_plan_create[_ex], srf_plan_rebuild and srf_batch_create (freed by the
+ *    matching *_destroy) and transient scratch in srf_sinogram_lors.
  *  - LORs are AoS rows of `lor_stride` floats [x1,y1,z1,x2,y2,z2,(tof,...)] in
  *    mm, exactly Lors.data of srfnef/data/lors.py:12-22 (stride 6 or 7).
  *  - TOF values are read from `tof_values[e * tof_stride]`; pass
@@ -60,8 +60,8 @@ typedef enum {
 
 /* Ratio kinds of the EM drivers (reference file:line per kind). */
 typedef enum {
-    SRF_RATIO_PLAIN = 0,      /* q = y / (p + eps): mlem.py:41-42 (eps = mean(y)*1e-8), osem_full.py:219-220 (1e-8) */
-    SRF_RATIO_DIV_NO_NAN = 1  /* q = (p+eps)==0 ? 0 : y/(p+eps): mlem_full.py:68 (eps 0), mlem_full_tof.py:109 (1e-5) */
+    SRF_RATIO_PLAIN = 0,      /* q = y / (p + eps): mlem.py:41-42 (eps = mean(y)*1e-8), osem_full.py:74-76 (1e-8) */
+    SRF_RATIO_DIV_NO_NAN = 1  /* q = (p+eps)==0 ? 0 : y/(p+eps): mlem_full.py:68 (eps 0), mlem_full_tof.py:64 (1e-5) */
 } srf_ratio_kind_t;
 
 #define SRF_OK 0
@@ -95,8 +95,8 @@ int srf_backproject(const float *lors, int64_t n, int64_t lor_stride,
                     float *image, void *stream);
 
 /* Backprojection of the EM ratio q[e] = ratio(y[e], p[e]) without
- * materialising q: the ratio of mlem.py:41-42 / mlem_full_tof.py:109 /
- * osem_full.py:219-220 fused into the backprojector's event load. */
+ * materialising q: the ratio of mlem.py:41-42 / mlem_full_tof.py:64 /
+ * osem_full.py:74-76 fused into the backprojector's event load. */
 int srf_backproject_ratio(const float *lors, int64_t n, int64_t lor_stride,
                           const float *tof_values, int64_t tof_stride,
                           const float *y, const float *p, srf_ratio_kind_t kind, float eps,
@@ -109,7 +109,7 @@ int srf_ratio(const float *y, const float *p, float *q, int64_t n,
               srf_ratio_kind_t kind, float eps, void *stream);
 
 /* EM multiplicative update x <- x * bp / e, e = emap (zero_guard == 0,
- * osem_full.py:221) or e = (emap == 0 ? zero_guard : emap) (mlem.py:33-35,43,
+ * osem_full.py:77) or e = (emap == 0 ? zero_guard : emap) (mlem.py:33-35,43,
  * zero_guard = 1e8).  16 B/voxel, one pass. */
 int srf_em_update(float *x, const float *bp, const float *emap, int64_t nvox,
                   float zero_guard, void *stream);
@@ -155,7 +155,7 @@ int srf_image_from_kernel_layout(const float *src_kernel, float *dst_xyz, int32_
  * (siddon.cu.cc:706-753, siddon_tof.cu.cc:761-812); results agree with them to
  * float round-off (summation order differs).
  *
- * srf_plan_create, srf_plan_rebuild and srf_plan_destroy are the only entry
+ * srf_plan_create, srf_plan_rebuild and srf_plan_destroy are the plan entry
  * points that allocate or free device memory (the binned records, about 64 B
  * per event-tile segment, plus per-item tile buffers); srf_plan_info reports
  * the total.  The LOR array is read during create / rebuild only, which
@@ -163,8 +163,8 @@ int srf_image_from_kernel_layout(const float *src_kernel, float *dst_xyz, int32_
  * segments (SRF_ERR_UNSUPPORTED beyond: split it into several plans, as OSEM
  * subsets are).  The memory comes from a stream-ordered pool per
  * device that keeps up to 16 GiB of freed plan memory for the next plans
- * (hipMalloc where a device has no pools); srf_plan_destroy first waits for
- * the device, as hipFree does. */
+ * (hipMalloc where a device has no pools; srf_plan_pool_trim releases it);
+ * srf_plan_destroy first waits for the device, as hipFree does. */
 typedef struct srf_plan srf_plan;
 
 int srf_plan_create(const float *lors, int64_t n, int64_t lor_stride,
@@ -194,20 +194,29 @@ int srf_plan_create_ex(const float *lors, int64_t n, int64_t lor_stride,
  * they are grown when the new list needs more and otherwise kept, with the
  * binning scratch, so a loop over chunks of one size (the sensitivity image
  * over all crystal pairs) allocates only on its first pass.  Arguments as
- * srf_plan_create_ex; on failure the plan holds an empty list.  No reference
- * counterpart (srfnef re-runs its TF op per call). */
+ * srf_plan_create_ex.  On failure the plan keeps its previous grid and holds
+ * an empty list: projections write nothing, backprojections write zeros (add
+ * nothing when accumulating), srf_plan_backproject_update sets x to
+ * x * 0 / emap, and srf_batch_create refuses it until a rebuild succeeds.
+ * No reference counterpart (srfnef re-runs its TF op per call). */
 int srf_plan_rebuild(srf_plan *plan, const float *lors, int64_t n, int64_t lor_stride,
                      const float *tof_values, int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof,
                      const srf_plan_opts_t *opts, void *stream);
 int srf_plan_destroy(srf_plan *plan);
+/* Returns the current device's pooled plan memory above keep_bytes to the
+ * system (hipMemPoolTrimTo after draining the device), so that other HIP users
+ * of the process (torch's caching allocator) can have it; 0 releases all
+ * memory no live plan holds.  No reference counterpart (the plan pool is this
+ * library's). */
+int srf_plan_pool_trim(size_t keep_bytes);
 int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments,
                   int64_t *n_items, size_t *device_bytes);
 /* out[e] = projection of event e (srf_project semantics). */
 int srf_plan_project(srf_plan *plan, const float *image, float *out, void *stream);
 /* q[e] = ratio(y[e], p[e]) with p = the projection (srf_plan_project then
  * srf_ratio, fused: the ratio is applied in the per-event reduction).  The EM
- * sub-iteration's forward half (mlem.py:40-42, mlem_full_tof.py:108-109,
- * osem_full.py:218-220). */
+ * sub-iteration's forward half (mlem.py:40-42, mlem_full_tof.py:63-64,
+ * osem_full.py:74-75). */
 int srf_plan_project_ratio(srf_plan *plan, const float *image, const float *y, srf_ratio_kind_t kind,
                            float eps, float *q, void *stream);
 /* image (+)= backprojection (srf_backproject semantics; values NULL = ones). */

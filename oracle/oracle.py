@@ -10,8 +10,8 @@ drivers with numpy float32 arithmetic:
 
   Mlem          srfnef/functions/mlem.py:28-45
   MlemFull      srfnef/functions/mlem_full.py:47-70
-  MlemFullTof   srfnef/functions/mlem_full_tof.py:88-112
-  OsemFull      srfnef/functions/osem_full.py:193-221 (subset loop)
+  MlemFullTof   srfnef/functions/mlem_full_tof.py:43-67
+  OsemFull      srfnef/functions/osem_full.py:49-77 (subset loop)
   attenuation   srfnef/corrections/attenuation/attenuation_correct.py:20-40
 
 Images passed to these functions are in the Python layout (nx, ny, nz); like
@@ -171,7 +171,7 @@ def mlem(values, lors, emap, center, size, n_iter, tof=None, driver="mlem"):
 
     driver='mlem'      srfnef/functions/mlem.py:33-43   q = y / (p + mean(y)*1e-8)
     driver='full'      srfnef/functions/mlem_full.py    q = div_no_nan(y, p)
-    driver='full_tof'  srfnef/functions/mlem_full_tof.py:101-112  q = div_no_nan(y, p + 1e-5)
+    driver='full_tof'  srfnef/functions/mlem_full_tof.py:62-67  q = div_no_nan(y, p + 1e-5)
     Each: x0 = 1; emap zeros -> 1e8; x <- x * BP(q) / emap.
     The backprojection is accumulated in double and rounded to float (the
     reference's atomicAdd order is unspecified)."""
@@ -206,13 +206,13 @@ def ratio(y, p, driver):
 
 def osem(values, lors, emap, center, size, n_iter, n_sub, perm, tof=None, driver="osem",
          reference_subset_order=False, guard_emap=False):
-    """OSEM restated from srfnef/functions/osem_full.py:204-221.
+    """OSEM restated from srfnef/functions/osem_full.py:60-77.
 
     perm: the event permutation (the reference shuffles with the unseeded global
-    RNG, osem_full.py:207-208; here it is passed in).  n_per_sub = N // n_sub
-    (osem_full.py:209).  Sub-iteration i uses subset i % n_sub, or i // n_sub when
-    reference_subset_order (osem_full.py:212).  The emap is neither guarded nor
-    scaled per subset (osem_full.py:210,221) unless guard_emap."""
+    RNG, osem_full.py:63-64; here it is passed in).  n_per_sub = N // n_sub
+    (osem_full.py:65).  Sub-iteration i uses subset i % n_sub, or i // n_sub when
+    reference_subset_order (osem_full.py:68).  The emap is neither guarded nor
+    scaled per subset (osem_full.py:66,77) unless guard_emap."""
     emap = np.array(emap, np.float32, copy=True)
     if guard_emap:
         emap[emap == 0.0] = 1e8

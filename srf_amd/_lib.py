@@ -98,6 +98,7 @@ SIGNATURES = [
     ("srf_plan_rebuild", ctypes.c_int, [_P, _P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.POINTER(SrfPlanOpts),
                                          _P]),
     ("srf_plan_destroy", ctypes.c_int, [_P]),
+    ("srf_plan_pool_trim", ctypes.c_int, [ctypes.c_size_t]),
     ("srf_batch_create", ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(SrfBatchScan), _I32,
                                         ctypes.POINTER(SrfBatchCfg), ctypes.POINTER(_P), _P]),
     ("srf_batch_destroy", ctypes.c_int, [_P]),

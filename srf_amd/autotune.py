@@ -6,7 +6,13 @@ work-item order.  Each candidate is timed on one OSEM subset of the actual
 batch (plans built with its item size, a warm-up step, then the median of
 `reps` timed steps); the fastest wins, ties going to the earlier candidate.
 Decisions are cached in a JSON file under a signature of the device and the
-workload, so the same workload gets the same schedule without re-timing.
+workload, so the same workload gets the same schedule without re-timing.  A
+cache file that cannot be read (truncated, not JSON, wrong structure) is
+ignored with a warning (and replaced at the next save) and the workload is
+timed again; an entry that names no
+candidate is dropped and re-timed; if timing itself fails, the default
+BatchConfig is used.  The file is written atomically (temporary file, then
+os.replace), so an interrupted write never leaves a partial cache.
 
 Every candidate computes the same reconstruction: the launch mode and order do
 not change any sum, and the item size only changes how the backprojector's
@@ -17,6 +23,7 @@ import json
 import math
 import os
 import time
+import warnings
 
 import torch
 
@@ -56,15 +63,42 @@ class Autotuner:
         self.candidates = tuple(candidates)
         self.reps = int(reps)
         self.cache = {}
+        self.cache_error = None
         if cache_path and os.path.exists(cache_path):
-            with open(cache_path) as f:
-                self.cache = json.load(f)
+            try:
+                with open(cache_path) as f:
+                    cache = json.load(f)
+                if not isinstance(cache, dict):
+                    raise ValueError("top level is not an object")
+                self.cache = cache
+            except (OSError, ValueError) as e:   # JSONDecodeError is a ValueError
+                self.cache_error = f"{type(e).__name__}: {e}"
+                warnings.warn(f"autotune cache {cache_path} unreadable ({self.cache_error}); timing again")
 
     def _save(self):
         if self.cache_path:
-            os.makedirs(os.path.dirname(os.path.abspath(self.cache_path)), exist_ok=True)
-            with open(self.cache_path, "w") as f:
+            path = os.path.abspath(self.cache_path)
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            tmp = f"{path}.tmp{os.getpid()}"
+            with open(tmp, "w") as f:
                 json.dump(self.cache, f, indent=1, sort_keys=True)
+            os.replace(tmp, path)
+
+    def _cached(self, key):
+        """The cached BatchConfig for key if the entry names a candidate."""
+        e = self.cache.get(key)
+        try:
+            c = BatchConfig(**e["config"])
+        except (TypeError, KeyError, ValueError):
+            if e is not None:
+                warnings.warn(f"autotune cache entry for {key!r} is malformed; timing again")
+                self.cache.pop(key, None)
+            return None
+        if c not in self.candidates:
+            warnings.warn(f"autotune cache entry for {key!r} names no candidate ({c}); timing again")
+            self.cache.pop(key, None)
+            return None
+        return c
 
     def _time_step(self, b):
         b.step(0)
@@ -99,12 +133,17 @@ class Autotuner:
         """BatchConfig for this workload (cached).  `timer(cand) -> seconds`
         replaces the GPU timing (tests)."""
         key = signature(scans, use_tof) + "".join(f"|{k}={kw[k]}" for k in sorted(kw))
-        if key in self.cache:
-            return BatchConfig(**self.cache[key]["config"])
-        if timer is not None:
-            timings = {i: float(timer(c)) for i, c in enumerate(self.candidates)}
-        else:
-            timings = self._gpu_timings(scans, n_sub, tof_bin, tof_sigma2, use_tof, log=log, **kw)
+        cached = self._cached(key)
+        if cached is not None:
+            return cached
+        try:
+            if timer is not None:
+                timings = {i: float(timer(c)) for i, c in enumerate(self.candidates)}
+            else:
+                timings = self._gpu_timings(scans, n_sub, tof_bin, tof_sigma2, use_tof, log=log, **kw)
+        except (RuntimeError, ValueError) as e:
+            warnings.warn(f"autotune timing failed ({type(e).__name__}: {e}); using the default schedule")
+            return BatchConfig()
         best = self.candidates[choose(timings, self.candidates)]
         self.cache[key] = {"config": best.__dict__.copy(),
                            "timings_ms": {str(self.candidates[i].__dict__): round(1e3 * v, 4)

@@ -10,6 +10,7 @@ same plan item size, because the batched kernels run the same per-item device
 code and keep every reduction's order (srf_batch_* in include/srf_hip.h).
 """
 import ctypes
+import warnings
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -93,6 +94,10 @@ class BatchOsem:
             self.plans[k] = plans
             self.batches[k] = self._make_batch(k, plans)
         self.graph = None
+        self.graph_error = None    # why capture failed (then iterations run eager)
+        self.done = 0              # iterations run so far
+        self.values_set = False    # set_values was called: checkpoints carry the values
+        self.tof = (bool(use_tof), float(tof_bin), float(tof_sigma2), float(tof_window))
 
     def _make_batch(self, k, plans):
         n = len(plans)
@@ -137,23 +142,69 @@ class BatchOsem:
             self.step(k)
 
     def capture(self):
-        """Capture one full iteration (every subset) in a HIP graph."""
+        """Capture one full iteration (every subset) in a HIP graph.  Capture
+        records the launches without running them, so a failed capture leaves
+        every estimate as it was; the error is kept in graph_error and raised."""
         torch.cuda.synchronize(self.dev)
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            self.iteration()
+        try:
+            with torch.cuda.graph(g):
+                self.iteration()
+        except RuntimeError as e:
+            self.graph = None
+            self.graph_error = str(e)
+            torch.cuda.synchronize(self.dev)
+            raise
         self.graph = g
         return g
 
-    def run(self, n_iter, graph=False):
-        if graph:
-            if self.graph is None:
+    def advance(self, graph=False):
+        """One iteration: the captured graph's replay when `graph` (capturing
+        it first), eager launches when not, or when capture failed (a warning
+        names the reason once; the result is the same either way)."""
+        if graph and self.graph is None and self.graph_error is None:
+            try:
                 self.capture()
-            for _ in range(n_iter):
-                self.graph.replay()
+            except RuntimeError as e:
+                warnings.warn(f"HIP graph capture failed, running eager launches instead: {e}")
+        if graph and self.graph is not None:
+            self.graph.replay()
         else:
-            for _ in range(n_iter):
-                self.iteration()
+            self.iteration()
+        self.done += 1
+
+    def checkpoint_key(self):
+        """What a checkpoint of this batch must match: every scan's events and
+        grid, OSEM constants, TOF and plan options, and the launch schedule (the
+        autotuned item size changes how partial sums are grouped)."""
+        return {"kind": "batch_osem", "n_sub": self.n_sub, "ratio_kind": self.ratio_kind, "eps": self.eps,
+                "zero_guard": self.zero_guard, "tof": list(self.tof), "config": dict(self.config.__dict__),
+                "scans": [[int(sc.lors.shape[0]), int(sc.seed), [int(v) for v in sc.shape], sc.geom]
+                          for sc in self.scans]}
+
+    def run(self, n_iter, graph=False, checkpoint=None):
+        """n_iter more iterations.  With `checkpoint` (a file path) a checkpoint
+        of this batch found there is resumed and one is written after every
+        iteration (estimates, and event values once set_values was used)."""
+        self.run_to(self.done + int(n_iter), graph, checkpoint)
+
+    def run_to(self, total, graph=False, checkpoint=None):
+        from . import checkpoint as ckpt
+        key = self.checkpoint_key() if checkpoint else None
+        if checkpoint:
+            state = ckpt.load(checkpoint, key)
+            if state is not None and state["done"] > self.done:
+                for x, v in zip(self.x, state["images"]):
+                    x.copy_(v.to(self.dev))
+                if "values" in state:
+                    for y, v in zip(self.y, state["values"]):
+                        y.copy_(v.to(self.dev))
+                    self.values_set = True
+                self.done = int(state["done"])
+        while self.done < total:
+            self.advance(graph)
+            if checkpoint:
+                ckpt.save(checkpoint, key, self.done, self.x, self.y if self.values_set else None)
 
     def reset(self):
         for x in self.x:
@@ -164,22 +215,28 @@ class BatchOsem:
         batch reads them at every step, so a captured graph sees the change."""
         pidx = self.perm[i][: self.n_per_sub[i] * self.n_sub]
         self.y[i].copy_(values.reshape(-1).to(self.dev, torch.float32).index_select(0, pidx))
+        self.values_set = True
 
     def image(self, i):
         """Scan i's estimate in (x, y, z) order."""
         return image_from_kernel(self.x[i])
 
-    def close(self):
+    def close(self, trim=True):
+        """Free the batches and plans; with `trim`, the plan pool's freed
+        memory goes back to the device (srf_plan_pool_trim) for other users."""
         torch.cuda.synchronize(self.dev)
         self.graph = None
         for k, h in list(self.batches.items()):
             if h:
                 lib().srf_batch_destroy(h)
         self.batches = {}
+        had_plans = bool(self.plans)
         for plans in self.plans.values():
             for p in plans:
                 p.close()
         self.plans = {}
+        if trim and had_plans:
+            ops.trim_plan_pool(0, self.dev)
 
     def __del__(self):
         try:

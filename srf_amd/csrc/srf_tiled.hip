@@ -336,18 +336,20 @@ __device__ __forceinline__ unsigned long long fixed_term(double v, float r)
 }
 
 // med3(x, lo, hi) = min(max(x, lo), hi) for lo <= hi, one v_med3_i32 (the
-// compiler emits it only for constant bounds), and a * b + c on the low 24
-// bits of a and b (v_mad_u32_u24).  Plain ALU instructions, no memory access.
+// compiler emits it only for constant bounds), and a * b + c with a and b taken
+// as signed 24-bit integers (v_mad_i32_i24: the clamped cell index is -1 on
+// the low-edge halo of a tile at the image origin, which the unsigned form
+// would read as 2^24 - 1).  Plain ALU instructions, no memory access.
 __device__ __forceinline__ int med3_i32(int x, int lo, int hi)
 {
     int r;
     asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
     return r;
 }
-__device__ __forceinline__ int mad_u24(int a, int b, int c)
+__device__ __forceinline__ int mad_i24(int a, int b, int c)
 {
     int r;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
     return r;
 }
 
@@ -458,10 +460,10 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const int cz1 = cvt_flr(zl), cz2 = cvt_flr(zh);
         const bool bs = cs1 != cs2, bz = cz1 != cz2;
         // LDS addresses of the three terms (dummy where the term is not
-        // applied); term 2 is i2a if rs > rz, else i2b.  (24-bit
-        // multiply-adds: a cell whose address is used lies in the tile or its
-        // halo, so its coordinates are below 2^24; logical operators keep the
-        // predicates as lane masks.)
+        // applied); term 2 is i2a if rs > rz, else i2b.  (Signed 24-bit
+        // multiply-adds: a clamped cell index lies in [origin - 1, origin +
+        // extent], within [-1, 2^23), and a byte stride is below 2^13;
+        // logical operators keep the predicates as lane masks.)
         const bool los = cs1 >= -1, loz = cz1 >= -1;
         int i1, i2a, i2b, i3;
         if (HALO) {
@@ -469,11 +471,11 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
             // address two 24-bit multiply-adds
             const int c1 = med3_i32(cs1, slo, shi), c2 = med3_i32(cs2, slo, shi);
             const int d1 = med3_i32(cz1, zlo, zhi), d2 = med3_i32(cz2, zlo, zhi);
-            const int A = mad_u24(c1, bs_v, base), B = mad_u24(c2, bs_v, base);
+            const int A = mad_i24(c1, bs_v, base), B = mad_i24(c2, bs_v, base);
             // computed before the selects: inline asm is not speculated, so a
             // multiply-add inside a select would become a branch
-            const int j3 = mad_u24(d2, bz_v, B), j2a = mad_u24(d2, bz_v, A), j2b = mad_u24(d1, bz_v, B);
-            i1 = mad_u24(d1, bz_v, A);
+            const int j3 = mad_i24(d2, bz_v, B), j2a = mad_i24(d2, bz_v, A), j2b = mad_i24(d1, bz_v, B);
+            i1 = mad_i24(d1, bz_v, A);
             i3 = ((bs || bz) && los && loz) ? j3 : dummy;
             i2a = (bs && bz && loz) ? j2a : dummy;
             i2b = (bs && bz && los) ? j2b : dummy;
@@ -885,6 +887,17 @@ __global__ __launch_bounds__(256) void k_tile_update(const float *__restrict__ t
     x[v] = x[v] * s / e;
 }
 
+// x[v] = x[v] * 0 / emap[v]: k_tile_update for a plan with no work items.
+__global__ __launch_bounds__(256) void k_update_empty(const float *__restrict__ emap, float guard,
+                                                      float *__restrict__ x, int64_t n)
+{
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= n) return;
+    float e = emap[v];
+    if (guard != 0.0f && e == 0.0f) e = guard;
+    x[v] = x[v] * 0.0f / e;
+}
+
 // Per item, the largest weight constant of its records in each mode.
 // The records' (Wbp, Wfp) pairs are read from the compact copy the
 // permutation writes (8 B per record instead of a 64 B record line).
@@ -1154,6 +1167,7 @@ struct srf_plan {
     bool tof, ds, dz;
     int64_t n, nseg;
     int nitems;
+    bool valid;   // the last build succeeded: item and tile tables match G / T
     DevBuf<SegRec> recs;
     DevBuf<int> rec_ev;
     DevBuf<Item> items;
@@ -1361,7 +1375,8 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     const unsigned item_max = (unsigned)item_records;
     if (n < 0 || (n > 0 && (!lors || lor_stride < 6))) return SRF_ERR_INVALID_ARG;
     if (tof && n > 0 && (!tof_values || tof_stride < 1)) return SRF_ERR_INVALID_ARG;
-    if (n >= ((int64_t)1 << 31)) return SRF_ERR_UNSUPPORTED;
+    // n + 1 event offsets are scanned as a 32-bit count
+    if (n >= (int64_t)INT_MAX) return SRF_ERR_UNSUPPORTED;
     LaunchGeom G;
     int rc = make_geom(grid, tof, G);
     if (rc) return rc;
@@ -1374,14 +1389,13 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     T.nkeys = T.ntiles * NCLASS;
     if (T.nkeys > MAX_KEYS) return SRF_ERR_UNSUPPORTED;
 
+    // the plan holds no list until this build succeeds; its grid and kernel
+    // variant are committed only then
     P->n = 0;
     P->nseg = 0;
     P->nitems = 0;
-    P->G = G;
-    P->T = T;
-    P->tof = tof != nullptr;
-    P->ds = G.dy != 1.0f;
-    P->dz = G.dz != 1.0f;
+    P->valid = false;
+    const bool use_tof = tof != nullptr;
     PB_TRY(ensure(P->ev_off, (size_t)n + 1, s));
     PB_TRY(ensure(P->ev_flag, (size_t)n, s));
     PB_TRY(ensure(P->ev_cnt, (size_t)n + 1, s));
@@ -1393,7 +1407,7 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     const unsigned nb = (unsigned)((n + BIN_THREADS - 1) / BIN_THREADS);
     const size_t shm = (size_t)T.nkeys * sizeof(unsigned);
     if (n > 0) {
-        if (P->tof)
+        if (use_tof)
             hipLaunchKernelGGL(k_bin_count<true>, dim3(nb), dim3(BIN_THREADS), shm, s, lors, n, (int)lor_stride,
                                tof_values, (int)tof_stride, G, T, twin, P->ev_cnt.p, P->ev_flag.p, P->key_count.p);
         else
@@ -1462,7 +1476,7 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
         PB_TRY(ensure(P->idx2, (size_t)nseg, s));
         PB_TRY(ensure(P->recs2, (size_t)nseg, s));
         PB_TRY(ensure(P->wpair, (size_t)nseg, s));
-        if (P->tof)
+        if (use_tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
                                tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->sk.p);
         else
@@ -1496,9 +1510,15 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
         hipLaunchKernelGGL(k_item_wmax, dim3((unsigned)nitems), dim3(256), 0, s, P->wpair.p, P->items.p);
     PB_HIP(hipGetLastError());
     PB_HIP(hipStreamSynchronize(s));
+    P->G = G;
+    P->T = T;
+    P->tof = use_tof;
+    P->ds = G.dy != 1.0f;
+    P->dz = G.dz != 1.0f;
     P->n = n;
     P->nseg = nseg;
     P->nitems = nitems;
+    P->valid = true;
     return SRF_OK;
 }
 
@@ -1541,8 +1561,12 @@ int srf_plan_rebuild(srf_plan *plan, const float *lors, int64_t n, int64_t lor_s
     if (rc) {
         (void)hipStreamSynchronize((hipStream_t)stream);
         (void)hipGetLastError();
+        // plan_build committed nothing: the plan keeps its previous grid and
+        // holds an empty list (projections write nothing, backprojections
+        // zeros); it cannot join a batch until a rebuild succeeds
         plan->n = plan->nseg = 0;
         plan->nitems = 0;
+        plan->valid = false;
     }
     return rc;
 }
@@ -1550,6 +1574,27 @@ int srf_plan_rebuild(srf_plan *plan, const float *lors, int64_t n, int64_t lor_s
 int srf_plan_destroy(srf_plan *plan)
 {
     plan_free(plan);
+    return SRF_OK;
+}
+
+int srf_plan_pool_trim(size_t keep_bytes)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= POOL_DEVICES) {
+        (void)hipGetLastError();
+        return SRF_ERR_LAUNCH;
+    }
+    hipMemPool_t pool;
+    {
+        std::lock_guard<std::mutex> lock(pool_mu);
+        pool = pools[dev];
+    }
+    if (!pool) return SRF_OK;   // no plan was built on this device
+    // memory freed in stream order becomes releasable once that work is done
+    if (hipDeviceSynchronize() != hipSuccess || hipMemPoolTrimTo(pool, keep_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return SRF_ERR_LAUNCH;
+    }
     return SRF_OK;
 }
 
@@ -1629,11 +1674,23 @@ int srf_plan_project_ratio(srf_plan *P, const float *image, const float *y, srf_
     return launch_status();
 }
 
+static int64_t plan_voxels(const srf_plan *P) { return (int64_t)P->G.nx * P->G.ny * P->G.nz; }
+
 static int plan_bp(srf_plan *P, const float *values, const float *y, const float *p, int kind, float eps,
                    srf_bp_mode_t mode, int accumulate, float *image, hipStream_t s)
 {
     if (!P || !image) return SRF_ERR_INVALID_ARG;
     if (mode != SRF_BP_REFERENCE && mode != SRF_BP_ADJOINT) return SRF_ERR_INVALID_ARG;
+    if (P->nitems == 0) {
+        // an empty list (also after a failed rebuild): the backprojection is
+        // zero, and no tile buffer or item table is read
+        if (!accumulate) {
+            const int64_t nv = plan_voxels(P);
+            hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, (unsigned *)image,
+                               0u, nv);
+        }
+        return launch_status();
+    }
     mark_before(P, s);
     if (P->nitems > 0)
         PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values, y, p, kind, eps,
@@ -1675,6 +1732,13 @@ int srf_plan_backproject_update(srf_plan *P, const float *values, float *x, cons
 {
     if (!P || !x || !emap) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
+    if (P->nitems == 0) {
+        // empty list: x * 0 / emap, k_tile_update's operations with a zero sum
+        const int64_t nv = plan_voxels(P);
+        hipLaunchKernelGGL(k_update_empty, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, emap, zero_guard, x,
+                           nv);
+        return launch_status();
+    }
     mark_before(P, s);
     if (P->nitems > 0)
         PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values,
@@ -1757,7 +1821,7 @@ int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int3
         return SRF_ERR_INVALID_ARG;
     for (int i = 0; i < n; ++i) {
         const srf_plan *P = plans[i];
-        if (!P || !scans[i].image || !scans[i].emap) return SRF_ERR_INVALID_ARG;
+        if (!P || !P->valid || !scans[i].image || !scans[i].emap) return SRF_ERR_INVALID_ARG;
         if (P->n > 0 && (!scans[i].y || !scans[i].q)) return SRF_ERR_INVALID_ARG;
     }
     hipStream_t s = (hipStream_t)stream;

@@ -122,6 +122,21 @@ class PetCylindricalListmodeTrans:
 
 
 @nef_class
+class PetListmodeTrans:
+    """Dispatch on the scanner type (functions/data_transform/__init__.py:40-47).
+    A scanner without crystals (an ideal cylinder) has nothing to snap to: the
+    listmode is returned unchanged (the reference returns None there)."""
+    scanner: object
+
+    def __call__(self, listmode: Listmode) -> Listmode:
+        if isinstance(self.scanner, S.PetEcatScanner):
+            return PetEcatListmodeTrans(self.scanner)(listmode)
+        if isinstance(self.scanner, S.PetCylindricalScanner):
+            return PetCylindricalListmodeTrans(self.scanner)(listmode)
+        return listmode
+
+
+@nef_class
 class Sinogram:
     """Crystal-pair sinogram with its scanner (data/sinogram.py:14-16)."""
     data: object
@@ -240,5 +255,6 @@ def sino2lm(scanner, sinogram, lors=None):
 
 
 __all__ = ["crystal_table", "crystal_indices", "snap_lors", "PetEcatListmodeTrans", "PetCylindricalListmodeTrans",
+           "PetListmodeTrans",
            "Sinogram", "PetEcatListmodeToSinogram", "PetEcatSinogramToListmode", "PetCylindricalListmodeToSinogram",
            "PetCylindricalSinogramToListmode", "get_all_lors_id", "lm2sino", "sino2lm"]

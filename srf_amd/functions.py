@@ -21,6 +21,7 @@ import numpy as np
 import torch
 
 from . import ops
+from .checkpoint import tensor_digest
 from .data import Emap, Image, ImageConfig, Listmode, Lors, TofConfig, nef_class
 
 _MODES = ("hip", "tf-eager", "tf")
@@ -201,25 +202,33 @@ class Mlem:
         return Image(eng.image().cpu().numpy(), list(self.emap.center), list(self.emap.size))
 
 
+def _default_emap(scanner, image_config):
+    """The all-crystal-pair sensitivity image for a crystal scanner (the
+    reference's 'block-full', mlem_full.py:39-45, mlem_full_tof.py:39-41); an
+    ideal cylinder (no crystals) gets the sampled one."""
+    from .scanner import PetCylindricalScanner, PetEcatScanner
+    crystals = isinstance(scanner, (PetEcatScanner, PetCylindricalScanner))
+    return EmapGenerator("block-full" if crystals else "uniform", scanner)(image_config)
+
+
 def _mlem_full(self, tof):
+    """MlemFull (mlem_full.py:47-99) and MlemFullTof (mlem_full_tof.py:43-99).
+    Only the TOF driver runs the scatter-corrected second reconstruction
+    (mlem_full_tof.py:71-92); in mlem_full.py that block is commented out
+    (:74-93), so MlemFull ignores scatter_corr, as the reference does."""
     if self.emap is None:
-        # mlem_full.py:39-41: EmapGenerator('block-full', scanner); an ideal
-        # cylinder (no crystals) gets the sampled sensitivity image
-        from .scanner import PetCylindricalScanner, PetEcatScanner
-        crystals = isinstance(self.scanner, (PetEcatScanner, PetCylindricalScanner))
-        object.__setattr__(self, "emap", EmapGenerator("block-full" if crystals else "uniform",
-                                                       self.scanner)(self.image_config))
+        object.__setattr__(self, "emap", _default_emap(self.scanner, self.image_config))
     lm = self.listmode
     if self.atten_corr is not None:
         lm = self.atten_corr(lm)
-    # mlem_full.py:68 div_no_nan(y, p); mlem_full_tof.py:109 div_no_nan(y, p + 1e-5)
+    # mlem_full.py:68 div_no_nan(y, p); mlem_full_tof.py:64 div_no_nan(y, p + 1e-5)
     eps = 1e-5 if tof is not None else 0.0
     eng = EmEngine(self.listmode.lors.data, lm.data, self.emap.data, self.image_config.shape,
                    _geom(self.image_config), tof=tof, ratio_kind=1, eps=eps, zero_guard=1e8)
     for _ in range(self.n_iter):
         eng.sub_iteration()
     x = Image(eng.image().cpu().numpy(), list(self.image_config.center), list(self.image_config.size))
-    if self.scatter_corr is not None:
+    if tof is not None and self.scatter_corr is not None:
         assert self.atten_corr is not None
         lm2 = self.scatter_corr(x, self.atten_corr.u_map, self.scanner, self.listmode)
         eng = EmEngine(self.listmode.lors.data, lm2.data, self.emap.data, self.image_config.shape,
@@ -264,13 +273,13 @@ class MlemFullTof:
 
 def subset_permutation(n, seed):
     """Seeded event permutation (the reference shuffles with the unseeded
-    global RNG, osem_full.py:207-208)."""
+    global RNG, osem_full.py:63-64)."""
     return np.random.default_rng(seed).permutation(n)
 
 
 class OsemEngine(EmEngine):
     """OSEM over contiguous subsets of a once-permuted, device-resident LOR
-    array (no per-sub-iteration host gather or upload, cf. osem_full.py:213-217);
+    array (no per-sub-iteration host gather or upload, cf. osem_full.py:69-73);
     with ``tiled`` each subset is binned into its own plan at construction."""
 
     def __init__(self, lors_data, values, emap_data, shape, geom, n_sub, perm, tof=None, ratio_kind=0,
@@ -279,7 +288,7 @@ class OsemEngine(EmEngine):
         y = to_gpu(values).reshape(-1)
         n = lors.shape[0]
         self.n_sub = int(n_sub)
-        self.n_per_sub = n // self.n_sub          # osem_full.py:209 (remainder dropped)
+        self.n_per_sub = n // self.n_sub          # osem_full.py:65 (remainder dropped)
         if isinstance(perm, torch.Tensor):
             pidx = perm[: self.n_per_sub * self.n_sub].to(lors.device, torch.long)
         else:
@@ -288,21 +297,65 @@ class OsemEngine(EmEngine):
         super().__init__(lors.index_select(0, pidx).contiguous(), y.index_select(0, pidx).contiguous(),
                          emap_data, shape, geom, tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard,
                          tiled=tiled, plan=False, item_records=item_records, tof_window=tof_window)
-        del lors, y
+        # the permutation, named by a cheap fingerprint (checkpoint keys)
+        step = max(1, pidx.numel() // 65536)
+        self.perm_tag = f"{int(pidx.numel())}:{int(pidx[::step].sum())}:" \
+                        f"{tensor_digest(torch.cat([pidx[:4096], pidx[-4096:]]))}"
+        del lors, y, pidx
         self.plans = [self.make_plan(self.subset(k)[0]) for k in range(self.n_sub)] if self.tiled else None
+        self.done = 0   # sub-iterations run so far (over every run call)
 
     def subset(self, k):
         a, b = k * self.n_per_sub, (k + 1) * self.n_per_sub
         return self.lors[a:b], self.y[a:b]
 
-    def run(self, n_iter, reference_subset_order=False):
-        for i in range(n_iter):
-            k = (i // self.n_sub) if reference_subset_order else (i % self.n_sub)
+    def checkpoint_key(self, reference_subset_order):
+        """What a checkpoint of this engine must match to be resumed: the
+        events (permutation, count, sum of values), grid, TOF, EM constants,
+        plan options and subset order."""
+        return {"kind": "osem", "n_sub": self.n_sub, "n_per_sub": self.n_per_sub, "perm": self.perm_tag,
+                "values_sum": float(self.y.double().sum()), "shape": list(self.shape), "geom": list(self.geom),
+                "tof": [bool(self.use_tof), self.tof_bin, self.tof_sigma2], "ratio_kind": int(self.ratio_kind),
+                "eps": self.eps, "zero_guard": self.zero_guard, "tiled": self.tiled,
+                "item_records": self.item_records, "tof_window": self.tof_window,
+                "reference_subset_order": bool(reference_subset_order)}
+
+    def run(self, n_iter, reference_subset_order=False, checkpoint=None, every=None):
+        """n_iter more sub-iterations (run_to(done + n_iter))."""
+        self.run_to(self.done + int(n_iter), reference_subset_order, checkpoint, every)
+
+    def run_to(self, total, reference_subset_order=False, checkpoint=None, every=None):
+        """Sub-iterations until `total` have been run.  Sub-iteration t (counted
+        over all calls) uses subset t % n_sub, or the reference's t // n_sub
+        (osem_full.py:68, quirk Q14).  With `checkpoint` (a file path): a
+        checkpoint of this workload found there is resumed (estimate and count
+        restored, if it is ahead of this engine), and one is written after every
+        `every` sub-iterations (default: each full iteration of n_sub, as the
+        reference saves each iteration) and at the end."""
+        from . import checkpoint as ckpt
+        every = self.n_sub if every is None else max(1, int(every))
+        key = self.checkpoint_key(reference_subset_order) if checkpoint else None
+        if checkpoint:
+            state = ckpt.load(checkpoint, key)
+            if state is not None and state["done"] > self.done:
+                self.x.copy_(state["images"][0].to(self.x.device))
+                self.done = int(state["done"])
+        while self.done < total:
+            t = self.done
+            k = (t // self.n_sub) if reference_subset_order else (t % self.n_sub)
             lors, y = self.subset(k)
-            if self.plans is not None:
+            if k >= self.n_sub:
+                # past the last subset the reference's slice of the permutation
+                # is empty (osem_full.py:69): its backprojection is zero and the
+                # update x * 0 / emap
+                ops.em_update(self.x, torch.zeros_like(self.x), self.emap_k, self.zero_guard)
+            elif self.plans is not None:
                 self.sub_iteration(y=y, plan=self.plans[k])
             else:
                 self.sub_iteration(lors, y)
+            self.done += 1
+            if checkpoint and (self.done % every == 0 or self.done == total):
+                ckpt.save(checkpoint, key, self.done, [self.x])
 
 
 @nef_class
@@ -322,13 +375,20 @@ class OsemFull:
     reference_subset_order: bool = False
 
     def __call__(self) -> Image:
+        """osem_full.py:49-87.  The event values are those of the listmode
+        snapped to crystal centres (PetListmodeTrans, :54) and attenuation
+        corrected along the snapped LORs (:56-58); the projections run along
+        the original LORs (:70-74).  Deviation: with emap None the reference
+        asks EmapGenerator for a 'ring' mode no generator has (:45-47, raises
+        NotImplementedError); here the default is MlemFull's all-pairs image."""
         if self.emap is None:
-            object.__setattr__(self, "emap", EmapGenerator("uniform", self.scanner)(self.image_config))
-        lm = self.listmode
+            object.__setattr__(self, "emap", _default_emap(self.scanner, self.image_config))
+        from .data_transform import PetListmodeTrans
+        lm = PetListmodeTrans(self.scanner)(self.listmode)
         if self.atten_corr is not None:
             lm = self.atten_corr(lm)
         perm = subset_permutation(len(self.listmode), self.seed)
-        # osem_full.py:219-221: q = y / (p + 1e-8), emap unguarded
+        # osem_full.py:74-77: q = y / (p + 1e-8), emap unguarded
         eng = OsemEngine(self.listmode.lors.data, lm.data, self.emap.data, self.image_config.shape,
                          _geom(self.image_config), self.n_sub, perm, ratio_kind=0, eps=1e-8, zero_guard=0.0)
         eng.run(self.n_iter, self.reference_subset_order)

@@ -250,6 +250,14 @@ def _(img_k):
 
 # -------------------------------------------------------------------- plans --
 
+def trim_plan_pool(keep_bytes=0, device_=None):
+    """Return the plan pool's freed memory above keep_bytes on a device to the
+    system (srf_plan_pool_trim)."""
+    dev = device_ if device_ is not None else torch.device("cuda", torch.cuda.current_device())
+    with torch.cuda.device(dev):
+        check(lib().srf_plan_pool_trim(int(keep_bytes)), "srf_plan_pool_trim")
+
+
 class Plan:
     """A binned event list (srf_plan_*): the LDS-tiled projector/backprojector.
 

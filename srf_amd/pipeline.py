@@ -224,7 +224,10 @@ def reconstruct_batched(scans, n_sub=8, n_iter=4, estimator=None, config=BatchCo
     b = BatchOsem(scans, n_sub, TOF_BIN, s2, True, config=config, zero_guard=ZERO_GUARD, tof_window=TOF_WINDOW)
     t_setup = time.time() - t0
     if graph:
-        b.capture()
+        try:
+            b.capture()
+        except RuntimeError:
+            pass   # b.advance runs eager launches and warns (graph_error)
     t_sc = 0.0
     for it in range(n_iter):
         if it >= 1 and scatter is not None:
@@ -239,10 +242,7 @@ def reconstruct_batched(scans, n_sub=8, n_iter=4, estimator=None, config=BatchCo
                 b.set_values(i, estimator.values(sc, sino))
             torch.cuda.synchronize()
             t_sc += time.time() - ts
-        if graph:
-            b.graph.replay()
-        else:
-            b.iteration()
+        b.advance(graph)
     imgs = [b.image(i).clone() for i in range(len(scans))]
     torch.cuda.synchronize()
     t = time.time() - t0

@@ -311,7 +311,7 @@ class ScatterCorrect:
     normalised so the subtracted total is scatter_fraction of the counts
     (scatter.py:52-53).  `scanner` is unused (the geometry field describes the
     virtual detectors); the fourth argument is the listmode, as
-    MlemFull/MlemFullTof pass it (mlem_full_tof.py:118)."""
+    MlemFull/MlemFullTof pass it (mlem_full_tof.py:73)."""
     low_energy: float = 400.0
     high_energy: float = 650.0
     resolution: float = 0.15

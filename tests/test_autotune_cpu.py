@@ -54,3 +54,47 @@ def test_signature_ignores_scan_order_and_rounds_grids():
     b = [_S(3000, (250, 256, 128)), _S(1000, (96, 96, 64))]
     assert signature(a, True, "dev") == signature(b, True, "dev")
     assert signature(a, False, "dev") != signature(a, True, "dev")
+
+
+@pytest.mark.parametrize("content", ["", "{\"a\": ", "[1, 2]", "not json at all"])
+def test_corrupt_cache_is_ignored_and_replaced(tmp_path, content):
+    """An unreadable cache file (empty, truncated, not an object, not JSON) is
+    ignored with a warning; the workload is timed again and the file rewritten
+    whole."""
+    from srf_amd.autotune import Autotuner, CANDIDATES
+    path = tmp_path / "cache.json"
+    path.write_text(content)
+    with pytest.warns(UserWarning, match="unreadable"):
+        t = Autotuner(str(path))
+    assert t.cache == {} and t.cache_error
+    scans = [_S(1000, (96, 96, 64))]
+    got = t.tune(scans, 8, 3.66, 600.0, True, timer=lambda c: 2.0 - 0.1 * (c == CANDIDATES[3]))
+    assert got == CANDIDATES[3]
+    d = json.load(open(path))
+    assert list(d.values())[0]["config"] == CANDIDATES[3].__dict__
+
+
+def test_malformed_entry_is_retimed(tmp_path):
+    from srf_amd.autotune import Autotuner, CANDIDATES, signature
+    scans = [_S(1000, (96, 96, 64))]
+    key = signature(scans, True)
+    path = tmp_path / "cache.json"
+    for entry in ({"config": {"item_records": 777}}, {"config": {"bogus": 1}}, {"nothing": 0}):
+        path.write_text(json.dumps({key: entry}))
+        with pytest.warns(UserWarning, match="timing again"):
+            got = Autotuner(str(path)).tune(scans, 8, 3.66, 600.0, True, timer=lambda c: 1.0)
+        assert got == CANDIDATES[0]
+        assert json.load(open(path))[key]["config"] == CANDIDATES[0].__dict__
+
+
+def test_timing_failure_falls_back_to_default(tmp_path):
+    from srf_amd.autotune import Autotuner
+    from srf_amd.batch import BatchConfig
+
+    def broken(c):
+        raise RuntimeError("HIP error")
+
+    with pytest.warns(UserWarning, match="default schedule"):
+        got = Autotuner(str(tmp_path / "c.json")).tune([_S(10, (96, 96, 64))], 8, 3.66, 600.0, True, timer=broken)
+    assert got == BatchConfig()
+    assert not (tmp_path / "c.json").exists()

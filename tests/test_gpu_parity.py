@@ -176,3 +176,28 @@ def test_nan_lors_and_empty(T):
     e = ops.project(_gpu(T, np.zeros((0, 6), np.float32)), ops.to_kernel_layout(_gpu(T, img)), [16, 16, 8],
                     [0, 0, 0, 32, 32, 16], False, 0.0, 0.0)
     assert e.numel() == 0
+
+
+@pytest.mark.parametrize("vox", [1.0, 2.0])
+def test_tof_width_closed_form(T, vox):
+    """The GPU's TOF Gaussian has variance tof_sigma2/dx_^2 + (tof_bin/dx_)^2/12
+    in voxels (siddon_tof.cu.cc:56-59) and integral tof_bin_: the moments of
+    its per-slab weights along an axis-aligned LOR, from projections of the
+    images 1, k and k^2 (test_oracle.tof_moments), against the closed form --
+    not against the oracle, so a slip shared by both would fail here."""
+    from srf_amd import ops
+    from test_oracle import tof_moments
+    tof_bin, sig2 = 12.0 * vox, 9.0 * vox * vox
+
+    def project(img, lor, center, size):
+        p = ops.project(_gpu(T, lor), ops.to_kernel_layout(_gpu(T, img)), list(img.shape),
+                        list(center) + list(size), True, tof_bin, sig2)
+        return p.double().cpu().numpy()
+
+    for tofv in (0.0, 30.0 * vox):
+        s0, mean, var = tof_moments(project, vox, tof_bin, sig2, tofv)
+        print(f"vox {vox} tof {tofv}: integral {s0 * 800.0 ** 2 / vox:.6f} mean {mean:.4f} variance {var:.5f}",
+              flush=True)
+        assert s0 * 800.0 * 800.0 / vox == pytest.approx(tof_bin / vox, rel=1e-5)
+        assert var == pytest.approx(21.0, rel=1e-4)
+        assert mean == pytest.approx(-0.5 - tofv / vox / 2.0, abs=1e-3)

@@ -255,7 +255,7 @@ def test_digest_is_order_independent_and_sensitive(orc):
 
 def test_em_drivers_closed_forms(orc):
     # one MLEM iteration from x=1 is B(y / A1) / emap (mlem.py:39-43);
-    # OSEM with one subset equals MLEM (osem_full.py:211-221 with n_sub=1).
+    # OSEM with one subset equals MLEM (osem_full.py:67-77 with n_sub=1).
     rng = np.random.default_rng(9)
     shape, vox = (12, 12, 6), 4.0
     size = [48.0, 48.0, 24.0]
@@ -310,3 +310,40 @@ def test_quirk_q22_opposite_slopes(orc):
         want = siddon_line_integral(img, center, size, l[0:3], l[3:6]) * dx_ * dx_ / L2
         off += abs(pe - want) > 1e-5 * want
     assert off >= 5  # every ray with a double-crossing slab of opposite slopes
+
+
+def tof_moments(project, vox, tof_bin, sig2, tofv):
+    """Mean and variance, in voxels, of the TOF weights along an axis-aligned
+    LOR (x through voxel centres, one term per slab of geometric weight
+    dx_/L^2): projections of the images 1, k - 128 and (k - 128)^2, k the x
+    index.  `project(image, lors)` returns float64 projections."""
+    shape = (256, 4, 4)
+    size = [256.0 * vox, 4.0 * vox, 4.0 * vox]
+    y = z = 0.5 * vox
+    lor = np.array([[-400.0 * vox, y, z, 400.0 * vox, y, z, tofv]], np.float32)
+    k = np.arange(256, dtype=np.float64) - 128.0
+    s = []
+    for g in (np.ones_like(k), k, k * k):
+        img = np.broadcast_to(g[:, None, None], shape).astype(np.float32)
+        s.append(float(project(img, lor, [0.0, 0.0, 0.0], size)[0]))
+    mean = s[1] / s[0]
+    return s[0], mean, s[2] / s[0] - mean * mean
+
+
+@pytest.mark.parametrize("vox", [1.0, 2.0])
+def test_tof_width_includes_bin_term(orc, vox):
+    # siddon_tof.cu.cc:56-59: the Gaussian's variance is
+    # tof_sigma2/dx_^2 + (tof_bin/dx_)^2/12 (the bin's uniform spread added to
+    # the timing spread).  A Gaussian sampled at unit spacing keeps its
+    # variance to within exp(-2 pi^2 var) and its integral to tof_bin_, so the
+    # moments of the per-slab TOF weights fix both terms: with the bin term
+    # dropped the variance would be 9, not 21.
+    tof_bin, sig2 = 12.0 * vox, 9.0 * vox * vox
+    for tofv in (0.0, 30.0 * vox):
+        s0, mean, var = tof_moments(lambda im, l, c, s: orc.project(im, l, c, s, tof=(tof_bin, sig2), f64=True),
+                                    vox, tof_bin, sig2, tofv)
+        L = 800.0   # LOR length in voxels; geometric weight dx_ / L^2 per slab
+        assert s0 * L * L / vox == pytest.approx(tof_bin / vox, rel=1e-5)   # integral of w_tof = tof_bin_
+        assert var == pytest.approx(9.0 + 12.0 * 12.0 / 12.0, rel=2e-5)   # sigma^2 + bin^2 / 12
+        # the centre sits tof/2 from the LOR midpoint towards p1 (tof = d(p2) - d(p1), Q21)
+        assert mean == pytest.approx(-0.5 - tofv / vox / 2.0, abs=1e-3)

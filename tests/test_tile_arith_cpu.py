@@ -10,7 +10,10 @@ exhaustively on the CPU over origins, in-image extents and cell indices:
   cell is a zero: a halo cell (-1 or TX after subtracting the origin) or a tile
   cell beyond the image edge, both staged as 0;
 - every clamped index lies inside the halo tile's range [-1, TX], so its LDS
-  word lies inside the tile.
+  word lies inside the tile;
+- the byte address formed from it by signed 24-bit multiply-adds
+  (v_mad_i32_i24) is that word's, also for tiles at the image origin where the
+  clamped index is -1.
 
 The backprojector's fixed-point terms: one double FMA onto 1.5 * 2^52 gives
 the rounded integer product.
@@ -65,6 +68,69 @@ def test_halo_words_inside_tile():
     words = (l[:, None, None] + 1) + hsy * (l[None, :, None] + 1) + hsz * (lz[None, None, :] + 1)
     assert words.min() == 0 and words.max() == hpad - 1
     assert len(np.unique(words)) == words.size     # distinct cells, distinct words
+
+
+def _wrap32(v):
+    return ((np.asarray(v, np.int64) + 2 ** 31) % 2 ** 32) - 2 ** 31
+
+
+def mad_i24(a, b, c):
+    """v_mad_i32_i24: sign-extended low 24 bits of a and b, product plus c,
+    low 32 bits."""
+    def sext24(v):
+        v = np.asarray(v, np.int64) & 0xFFFFFF
+        return np.where(v >= 2 ** 23, v - 2 ** 24, v)
+    return _wrap32(sext24(a) * sext24(b) + np.asarray(c, np.int64))
+
+
+def mad_u24(a, b, c):
+    """v_mad_u32_u24: zero-extended low 24 bits of a and b."""
+    return _wrap32((np.asarray(a, np.int64) & 0xFFFFFF) * (np.asarray(b, np.int64) & 0xFFFFFF)
+                   + np.asarray(c, np.int64))
+
+
+def halo_addresses(xdom, p0, s0, z0, es, ez, mad):
+    """Byte addresses the projector's halo marcher forms for every slab of the
+    tile's primary range and every clamped (secondary, axial) cell, as
+    march_tile computes them: base folded with the tile origin, then two
+    24-bit multiply-adds.  Returns (addresses, expected word * 4)."""
+    hx, hy, hz = TX + 2, TX + 2, TZ + 2
+    hsy = hx + 1
+    hsz = hsy * hy + 1
+    es_b = 4
+    sp = 1 if xdom else hsy
+    ss = hsy if xdom else 1
+    bs, bz = ss * es_b, hsz * es_b
+    lp = np.arange(0, TX)
+    a = p0 + lp                                   # slab index
+    base = (a - p0 + 1) * sp * es_b - (s0 - 1) * bs - (z0 - 1) * bz
+    c = np.arange(s0 - 1, s0 + es + 1)            # clamp range along secondary
+    d = np.arange(z0 - 1, z0 + ez + 1)            # and along z
+    A = mad(c[None, :, None], bs, base[:, None, None])
+    addr = mad(d[None, None, :], bz, A)
+    word = (lp[:, None, None] + 1) * sp + (c[None, :, None] - s0 + 1) * ss + (d[None, None, :] - z0 + 1) * hsz
+    return addr, word * 4
+
+
+def test_halo_addresses_in_tile_at_image_origin():
+    """Every address the halo marcher forms, for tiles at the image origin
+    (s0 = 0, z0 = 0: the clamped index is -1) and elsewhere, is the intended
+    halo word and lies inside the tile's LDS words.  The unsigned 24-bit form
+    maps index -1 to 2^24 - 1 and leaves the tile."""
+    hx, hy, hz = TX + 2, TX + 2, TZ + 2
+    hsy = hx + 1
+    hsz = hsy * hy + 1
+    hpad = hsz * (hz - 1) + hsy * (hy - 1) + hx
+    for xdom in (True, False):
+        for n_s, n_z in ((256, 128), (96, 64), (50, 13)):
+            for s0 in range(0, n_s, TX):
+                for z0 in range(0, n_z, TZ):
+                    es, ez = min(TX, n_s - s0), min(TZ, n_z - z0)
+                    addr, want = halo_addresses(xdom, 64, s0, z0, es, ez, mad_i24)
+                    assert np.array_equal(addr, want)
+                    assert addr.min() >= 0 and addr.max() < hpad * 4
+    addr, want = halo_addresses(True, 0, 0, 0, TX, TZ, mad_u24)
+    assert addr.max() >= 2 ** 24 and not np.array_equal(addr, want)
 
 
 def test_fixed_point_term_is_rounded_product():
