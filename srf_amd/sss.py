@@ -20,6 +20,7 @@ the physical line integral of mu (the reference's UmapProject scaling
 L^2/unit_size[0] is exact only for 1 mm voxels, quirk Q15).
 """
 import math
+import os
 
 import numpy as np
 import torch
@@ -48,7 +49,13 @@ def energy_tables(low, high, resolution, physical_kn=False):
 
 
 def sss_scale(low, high, resolution):
-    """5/sqrt(1022 pi)/resolution * eff(511) / (4 pi) (scatter_fraction.py:27-33)."""
+    """5/sqrt(1022 pi)/resolution * eff(511) / (4 pi).  The reference's constant
+    (scatter_fraction.py:27-33) is eff_ws * area^2 * 5/sqrt(1022 pi)/resolution
+    / (4 pi) with eff_ws = eff(511) * 5/sqrt(1022 pi)/resolution: this one drops
+    a factor 5/sqrt(1022 pi)/resolution and the crystal area squared (virtual
+    detectors have no crystal area).  Both cancel wherever the estimate is
+    normalised (ScatterCorrect divides by its sum); the crystal-pair path
+    (srf_amd.scatter_pairs) applies the full constant."""
     s = math.sqrt(511.0) * resolution
     Ei = np.arange(int(low), int(high), 5, dtype=np.float64)
     e511 = float(np.sum(np.exp(-(Ei - 511.0) ** 2 / 2.0 / s ** 2)))
@@ -306,12 +313,20 @@ class SingleScatterSimulation:
 class ScatterCorrect:
     """Scatter correction with srfnef's fields (scatter.py:18-55).
 
-    __call__(emission_image, u_map, scanner, listmode) -> Listmode: the
-    listmode values minus the SSS estimate at each event's sinogram bin,
-    normalised so the subtracted total is scatter_fraction of the counts
-    (scatter.py:52-53).  `scanner` is unused (the geometry field describes the
-    virtual detectors); the fourth argument is the listmode, as
-    MlemFull/MlemFullTof pass it (mlem_full_tof.py:73)."""
+    __call__(emission_image, u_map, scanner, outdir, out_append=None) -> Listmode,
+    by the type of the fourth argument:
+      * a directory (str) holding scatter_preprocess's files, or the dict that
+        srf_amd.scatter_pairs.scatter_preprocess returns: the reference's
+        crystal-pair correction (scatter.py:25-55; srf_amd.scatter_pairs): the
+        counted crystal pairs' sinogram minus the normalised per-pair SSS,
+        times each pair's scale over its 1.3 mu attenuation, as a listmode of
+        crystal-centre LORs;
+      * a Listmode: the listmode values minus the SSS estimate at each event's
+        bin of a coarse virtual-detector sinogram (the geometry field),
+        normalised so the subtracted total is scatter_fraction of the counts
+        (scatter.py:52-53); `scanner` is unused.  MlemFullTof passes its
+        listmode there (mlem_full_tof.py:73), where the reference's np.load of
+        outdir + '/lors.npy' would fail."""
     low_energy: float = 400.0
     high_energy: float = 650.0
     resolution: float = 0.15
@@ -328,7 +343,13 @@ class ScatterCorrect:
                              dtype=torch.float32)
         return sss(em, mu), g
 
-    def __call__(self, emission_image, u_map, scanner=None, listmode=None):
+    def __call__(self, emission_image, u_map, scanner=None, outdir=None, out_append=None):
+        if isinstance(outdir, (str, os.PathLike, dict)):
+            from . import scatter_pairs as sp
+            pre = outdir if isinstance(outdir, dict) else sp.load_preprocessed(outdir, out_append)
+            frac, scale = sp.pair_scatter(pre, u_map, scanner, self.low_energy, self.high_energy, self.resolution)
+            return sp.correct_pairs(pre, frac, scale, self.scatter_fraction)[1]
+        listmode = outdir
         sino, g = self.estimate(emission_image, u_map)
         lors = torch.as_tensor(np.asarray(listmode.lors.data) if not isinstance(listmode.lors.data, torch.Tensor)
                                else listmode.lors.data).to(sino.device, torch.float32)
