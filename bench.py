@@ -32,6 +32,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """CPU threads for the baselines: every core of this process's affinity,
+    capped by OMP_NUM_THREADS where the environment sets it (the GPU host's
+    CPU share for one GPU's job)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +62,9 @@ def parse():
     ap.add_argument("--batch-seed", type=int, default=5)
     ap.add_argument("--batch-only", action="store_true", help="only the batched many-scan reconstruction")
     ap.add_argument("--no-emap", action="store_true", help="skip the crystal-pair sensitivity image")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the MLEM (64x64x32, 1e6 LORs) and differentiable-projector (8 x 128x128x64, "
+                         "4e6 LORs) GPU / CPU timings")
     ap.add_argument("--tof-window", type=float, default=22.0,
                     help="plan TOF window (terms below 2^-w of the event's TOF peak are not binned; 0 = all terms)")
     return ap.parse_args()
@@ -247,6 +259,10 @@ def main():
     scatter = None if args.no_scatter else scatter_bench(args)
     batch = None if args.no_batch else batch_bench(args)
     emap = None if args.no_emap else emap_bench(args)
+    configs = None if args.no_configs else config_bench(args)
+    # the headline workload end to end over 3 iterations, binning included
+    e2e = {"iterations": 3, "binning_s": plan_s, "osem_s": 3 * t / args.steps,
+           "events_per_s_with_binning": 3 * n_used / (plan_s + 3 * t / args.steps)}
 
     out = {"metric": METRIC, "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": t / args.steps * 1e3, "higher_is_better": True,
@@ -258,7 +274,8 @@ def main():
                       "kernels": "event-per-lane" if args.event_kernels else "lds-tiled plans",
                       "tof_window": 0.0 if args.event_kernels else args.tof_window,
                       "setup_binning_s": plan_s, "plan_subset0": plan_info},
-           "roofline": roof, "cpu_baseline": cpu, "scatter": scatter, "batch": batch, "emap": emap}
+           "roofline": roof, "cpu_baseline": cpu, "end_to_end_3_iterations": e2e, "scatter": scatter,
+           "batch": batch, "emap": emap, "configs": configs}
     print(json.dumps(out), flush=True)
 
 
@@ -309,7 +326,7 @@ def emap_bench(args):
         m = 1 << 20   # the first 2^20 pairs: crystals 0-32 against every crystal
         k = np.arange(m)
         lors = np.hstack((pos[k // n], pos[k % n])).astype(np.float32)
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        threads = host_threads()
         t = time.perf_counter()
         oracle.backproject(np.ones(m, np.float32), lors, (200, 200, 200), [0.0, 0.0, 0.0], [410.0, 410.0, 410.0],
                            f64=False, threads=threads)
@@ -421,13 +438,95 @@ def batch_bench(args):
             "surrogate_weights": os.path.basename(wpath), "tof_window": TOF_WINDOW}
 
 
+def config_bench(args):
+    """The other two configurations, GPU and CPU (C oracle, host_threads()):
+    non-TOF MLEM (Mlem semantics) of 1e6 LORs on 64x64x32 x 3.4 mm, 10
+    iterations, in full on both; the differentiable projector's forward and
+    backward (adjoint backprojector) of 4e6 TOF LORs on 128x128x64 x 2 mm, all
+    8 images on the GPU, one image in full on the CPU."""
+    import numpy as np
+    import torch
+    from srf_amd import ops, phantom
+    from srf_amd.autograd import Projector
+    from srf_amd.functions import EmEngine, uniform_cylinder_lors
+    dev = torch.device("cuda", 0)
+    threads = host_threads()
+    out = {}
+    # --- MLEM, 64x64x32 x 3.4 mm, 1e6 LORs, 10 iterations
+    act, center, size = phantom.cylinder_spheres_phantom((64, 64, 32), 3.4)
+    lors = phantom.sample_events(act, center, size, 1_000_000, 300.0, 110.0, seed=21, device=dev)
+    elors = phantom.sample_events(torch.ones_like(act), center, size, 1_000_000, 300.0, 110.0, seed=22, device=dev)
+    geom = list(center) + list(size)
+    emap = ops.from_kernel_layout(ops.backproject(elors, None, [64, 64, 32], geom, False, 0.0, 0.0, 0))
+    y = torch.ones(lors.shape[0], device=dev)
+    eps = float(np.float32(float(y.mean()) * 1e-8))
+    eng = EmEngine(lors, y, emap, (64, 64, 32), geom, ratio_kind=0, eps=eps, zero_guard=1e8)
+    eng.sub_iteration()                      # warm-up (code objects); then from x = 1 again
+    eng.x.fill_(1.0)
+    _, t_gpu = _timed(lambda: [eng.sub_iteration() for _ in range(10)])
+    x_gpu = eng.image().cpu().numpy()
+    res = {"workload": "non-TOF list-mode MLEM, R 300 mm, axial 110 mm, 64x64x32 x 3.4 mm, water cylinder + 2 "
+                       "hot spheres, 1e6 LORs, 10 iterations (Mlem semantics)",
+           "gpu_s": t_gpu, "gpu_events_per_s": 1e7 / t_gpu}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        ln, en = lors.cpu().numpy(), emap.cpu().numpy()
+        t = time.perf_counter()
+        x_cpu = oracle.mlem(y.cpu().numpy(), ln, en, center, size, 10, driver="mlem", threads=threads)
+        dt = time.perf_counter() - t
+        m = en > 0
+        res.update({"cpu_s": dt, "cpu_events_per_s": 1e7 / dt, "cores": threads, "kind": "port",
+                    "image_rel_l2_gpu_vs_cpu": float(np.linalg.norm((x_gpu - x_cpu)[m]) / np.linalg.norm(x_cpu[m]))})
+    out["mlem_64x64x32_1e6"] = res
+    log(f"configs: MLEM GPU {t_gpu:.3f} s" + (f", CPU {res['cpu_s']:.1f} s" if "cpu_s" in res else ""))
+    del eng, lors, elors
+    # --- differentiable projector, 8 x 128x128x64 x 2 mm, 4e6 TOF LORs
+    n = 4_000_000
+    L = uniform_cylinder_lors(n, 300.0, 300.0, 21, dev)
+    g = torch.Generator(device=dev).manual_seed(22)
+    tofv = torch.randn(n, 1, device=dev, generator=g) * 2 * 25.48
+    L = torch.cat([L, tofv], 1).contiguous()
+    shape, c3, s3, tof = (128, 128, 64), [0.0, 0.0, 0.0], [256.0, 256.0, 128.0], (3.66, 649.2)
+    imgs = phantom.smooth_lognormal_images(8, shape, 2.0, 5, device=dev)
+    A, t_plan = _timed(lambda: Projector(L, shape, c3, s3, tof=tof))
+    yv = torch.rand(n, device=dev, generator=g)
+
+    def fwd_bwd():
+        for b in range(8):
+            x = imgs[b].clone().requires_grad_(True)
+            (A(x) * yv).sum().backward()
+
+    fwd_bwd()
+    _, t_fb = _timed(fwd_bwd)
+    res = {"workload": "8 images 128x128x64 x 2 mm (smoothed lognormal), 4e6 TOF LORs with uniform endpoint "
+                       "pairs on a 300 mm cylinder, forward (projector) + backward (adjoint backprojector)",
+           "gpu_s": t_fb, "gpu_events_per_s": 8 * n / t_fb, "plan_s": t_plan}
+    if not args.no_cpu:
+        ln = L.cpu().numpy()
+        x0 = imgs[0].cpu().numpy()
+        t = time.perf_counter()
+        p = oracle.project(x0, ln, c3, s3, tof=tof, threads=threads)
+        oracle.backproject(yv.cpu().numpy(), ln, shape, c3, s3, tof=tof, variant=oracle.FP, f64=False,
+                           threads=threads)
+        dt = time.perf_counter() - t
+        del p
+        res.update({"cpu_s_one_image": dt, "cpu_events_per_s": n / dt, "cores": threads, "kind": "port",
+                    "cpu_sample": "image 0 of 8: forward + backward of all 4e6 LORs"})
+    out["autograd_8x128x128x64_4e6"] = res
+    log(f"configs: projector fwd+bwd GPU {t_fb:.3f} s for 8 images"
+        + (f", CPU {res['cpu_s_one_image']:.1f} s for one" if "cpu_s_one_image" in res else ""))
+    A.plan.close()
+    return out
+
+
 def cpu_baseline(eng, shape, center, size, tof_bin, tof_sigma2, sample):
     """The C oracle (oracle/srf_oracle.c, OpenMP) on the host cores: TOF
     projection + backprojection of the first `sample` events of subset 0."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = host_threads()
     l0, y0 = eng.subset(0)
     m = min(sample, l0.shape[0])
     lors = l0[:m].cpu().numpy()

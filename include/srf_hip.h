@@ -178,11 +178,14 @@ typedef struct {
     int32_t item_records;
     /* TOF window: 0 keeps every term, as the reference (siddon_tof.cu.cc
      * weights every slab, :55-59).  w > 0 bins only the slabs whose TOF
-     * Gaussian factor is at least 2^-w of its peak (plus one slab each side):
-     * The dropped weight is erfc(sqrt(w ln 2)) of an event's TOF weight:
-     * w = 22 drops < 4e-8 (below one float32 ulp of the total) and about half
-     * of the terms at 60 mm FWHM in a 512 mm field of view; w = 26, < 2e-9.
-     * Events whose window misses the image project to 0. */
+     * Gaussian factor is at least 2^-w of the largest factor among the
+     * event's slabs in the image (plus one slab each side).  Where the TOF
+     * centre lies in the image the dropped weight is erfc(sqrt(w ln 2)) of the
+     * event's: w = 22 drops < 4e-8 (below one float32 ulp of the total) and
+     * about half of the terms at 60 mm FWHM in a 512 mm field of view; w = 26,
+     * < 2e-9.  Where it lies outside, the window keeps the in-image tail
+     * around the largest term, so the EM ratio y / sum(terms) still spreads
+     * the event as the reference does. */
     float tof_window;
 } srf_plan_opts_t;
 

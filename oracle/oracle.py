@@ -166,7 +166,7 @@ def kernel_index_to_xyz(v, shape):
 
 # ---------------------------------------------------------------- drivers --
 
-def mlem(values, lors, emap, center, size, n_iter, tof=None, driver="mlem"):
+def mlem(values, lors, emap, center, size, n_iter, tof=None, driver="mlem", threads=0):
     """EM loop restated from the reference drivers, in numpy float32.
 
     driver='mlem'      srfnef/functions/mlem.py:33-43   q = y / (p + mean(y)*1e-8)
@@ -180,9 +180,9 @@ def mlem(values, lors, emap, center, size, n_iter, tof=None, driver="mlem"):
     x = np.ones(emap.shape, np.float32)
     y = np.asarray(values, np.float32)
     for _ in range(n_iter):
-        p = project(x, lors, center, size, tof=tof)
+        p = project(x, lors, center, size, tof=tof, threads=threads)
         q = ratio(y, p, driver)
-        bp = backproject(q, lors, emap.shape, center, size, tof=tof).astype(np.float32)
+        bp = backproject(q, lors, emap.shape, center, size, tof=tof, threads=threads).astype(np.float32)
         x = x * bp / emap
     return x
 

@@ -155,18 +155,28 @@ template <bool TOF>
 __device__ __forceinline__ void tof_coeffs_d(const Branch &B, const RaySetup &r, const LaunchGeom &G, double &tA,
                                              double &tB);
 
-// TOF window (twin > 0): keep the slabs whose Gaussian factor
-// exp2(-(pp1 tA + tB)^2) is at least 2^-twin of its peak, plus one slab of
-// margin on each side for the rounding of the float coefficients.
+// TOF window (twin > 0): of the slabs [lo, hi), keep those whose Gaussian
+// factor exp2(-(pp1 tA + tB)^2) is at least 2^-twin of the largest factor in
+// [lo, hi) -- the peak inside the image, not the Gaussian's own peak, which
+// lies outside for events whose TOF centre is outside the image: the EM ratio
+// y / sum(terms) spreads such an event over its tail terms, so they must stay
+// -- plus one slab of margin on each side for the rounding of the float
+// coefficients.  u = pp1 tA + tB is linear in the slab, so its smallest
+// magnitude over the range is 0 if it changes sign there, else the smaller
+// magnitude at the two ends (over the continuous range: a lower bound, so the
+// kept set is a superset).
 template <bool TOF>
 __device__ __forceinline__ void tof_window(const Branch &B, const RaySetup &r, const LaunchGeom &G, float twin,
                                            int &lo, int &hi)
 {
-    if (!TOF || !(twin > 0.0f)) return;
+    if (!TOF || !(twin > 0.0f) || lo >= hi) return;
     double tA, tB;
     tof_coeffs_d<TOF>(B, r, G, tA, tB);
     if (!(fabs(tA) > 0.0) || !isfinite(tA) || !isfinite(tB)) return;
-    const double D = sqrt((double)twin);
+    const double ua = ((double)lo - (double)B.np2) * tA + tB;
+    const double ub = ((double)(hi - 1) - (double)B.np2) * tA + tB;
+    const double umin = (ua <= 0.0) != (ub <= 0.0) ? 0.0 : fmin(fabs(ua), fabs(ub));
+    const double D = sqrt(umin * umin + (double)twin);
     const double u1 = (-D - tB) / tA, u2 = (D - tB) / tA;
     const double np2 = B.np2;
     lo = max(lo, dbl_to_int(ceil(fmin(u1, u2) + np2)) - 1);
@@ -702,31 +712,124 @@ __device__ __forceinline__ float event_value(const float *__restrict__ values, c
 // item has at most ITEM = 2^13 segments, so no voxel sum can reach 2^62.
 // Items holding a non-finite value use float LDS atomics instead (NaN and inf
 // then propagate as through the reference's atomicAdd).
+//
+// One scale resolves a term to 2^-50 of the item's largest; an item whose
+// event values span more than 2^BAND_BITS (the EM ratio y / (p + eps) of an
+// event whose projection is ~0 is 1e8 times the others') would lose its small
+// terms.  Its records are then accumulated in exponent bands: band b holds the
+// values in [2^(e - 16 (b + 1)), 2^(e - 16 b)), e the exponent of the largest,
+// each band with its own scale in its own pass over the records, and the
+// bands' float tiles are summed in band order (still bit-reproducible).  An
+// item within one band -- every item of ordinary data -- takes the single pass.
 constexpr int FX_BITS = 49;
+constexpr int BAND_BITS = 16;
+constexpr int BANDS_GRID = 256;   // workgroups of k_bp_bands (wide-range items are rare)
+constexpr int MAX_BANDS = 8;   // values below 2^-128 of the largest join the last band
 constexpr int RECS_PER_THREAD = ITEM / TILE_THREADS;
 static_assert(ITEM % TILE_THREADS == 0, "whole records per thread");
 
-// Backprojection of one work item into the LDS tile fx (shared flags in
-// s_cmax / s_bad), written to out[0, TVOX) as float.
+// Band of a nonzero value v for an item whose largest value is below 2^etop.
+__device__ __forceinline__ int value_band(float v, int etop, int nb)
+{
+    int ev = 0;
+    (void)frexpf(v, &ev);
+    return min((etop - ev) / BAND_BITS, nb - 1);
+}
+
+// Shift that scales band `band` of an item (largest value vtop < 2^etop,
+// weight constant wmax) below 2^FX_BITS.
+__device__ __forceinline__ int band_shift(float vtop, float wmax, int etop, int band)
+{
+    float c = band == 0 ? vtop * wmax : ldexpf(wmax, etop - BAND_BITS * band);
+    if (!isfinite(c)) c = 3.40282347e38f;
+    int ex = 0;
+    (void)frexpf(c, &ex);   // c < 2^ex
+    return FX_BITS - ex;
+}
+
+// Bands 1 .. nb-1 of a wide-range item, added to the band-0 tile in `out`
+// (k_bp_bands, for the items k_bp_tiled listed).  The item's values are read
+// again and reduced again in s_vmax / s_vmin.
+template <bool TOF, bool DS, bool DZ>
+__device__ __forceinline__ void bp_item_bands(const SegRec *__restrict__ recs, const Item &it,
+                                              const float *__restrict__ values, const float *__restrict__ y,
+                                              const float *__restrict__ p, int kind, float eps, int adjoint,
+                                              const LaunchGeom &G, const TileCfg &T,
+                                              const int *__restrict__ rec_ev, float *__restrict__ out,
+                                              unsigned long long *fx, unsigned *s_vmax, unsigned *s_vmin)
+{
+    float *fl = reinterpret_cast<float *>(fx);
+    int x0, y0, z0;
+    tile_origin(it.tile, T, x0, y0, z0);
+    const float wmax = adjoint ? it.wmax_fp : it.wmax_bp;
+    const int end = it.start + it.count;
+    __syncthreads();   // the previous item's tile and reductions are done with
+    if (threadIdx.x == 0) {
+        *s_vmax = 0u;
+        *s_vmin = 0x7f800000u;
+    }
+    __syncthreads();
+    float vmax = 0.0f, vmin = INFINITY;
+    for (int r = it.start + threadIdx.x; r < end; r += TILE_THREADS) {
+        const float v = fabsf(event_value(values, y, p, kind, eps, rec_ev[r]));
+        vmax = fmaxf(vmax, v);
+        if (v != 0.0f) vmin = fminf(vmin, v);
+    }
+    if (vmax > 0.0f) atomicMax(s_vmax, __float_as_uint(vmax));
+    if (vmin < INFINITY) atomicMin(s_vmin, __float_as_uint(vmin));
+    __syncthreads();
+    const float vtop = __uint_as_float(*s_vmax);
+    int etop = 0, ebot = 0;
+    (void)frexpf(vtop, &etop);
+    (void)frexpf(__uint_as_float(*s_vmin), &ebot);
+    const int nb = min((etop - ebot) / BAND_BITS + 1, MAX_BANDS);
+    for (int band = 1; band < nb; ++band) {
+        const int shift = band_shift(vtop, wmax, etop, band);
+        for (int i = threadIdx.x; i < TPAD; i += TILE_THREADS) fx[i] = 0ull;
+        __syncthreads();
+        for (int r = it.start + threadIdx.x; r < end; r += TILE_THREADS) {
+            const float v = event_value(values, y, p, kind, eps, rec_ev[r]);
+            if (v == 0.0f || value_band(v, etop, nb) != band) continue;
+            const SegRec R = recs[r];
+            const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
+            const int a = (int)(ab & 0xffffu), b = (int)(ab >> 16);
+            if (a >= b) continue;
+            TileWalk t;
+            walk_for(t, fl, TPAD, x0, y0, z0);
+            t.fx = fx;
+            t.value = ldexpf(v, shift);
+            march_any<TOF, DS, DZ, M_SCATTER_FX>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS)
+            out[i] = out[i] + ldexpf((float)(long long)fx[tile_word(i)], -shift);
+        __syncthreads();
+    }
+}
+
+// Backprojection of one work item into the LDS tile fx (shared reductions in
+// s_vmax / s_vmin / s_bad), written to out[0, TVOX) as float.
 template <bool TOF, bool DS, bool DZ>
 __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const Item &it,
                                         const float *__restrict__ values, const float *__restrict__ y,
                                         const float *__restrict__ p, int kind, float eps, int adjoint,
                                         const LaunchGeom &G, const TileCfg &T, const int *__restrict__ rec_ev,
-                                        float *__restrict__ out, unsigned long long *fx, unsigned *s_cmax,
-                                        int *s_bad)
+                                        float *__restrict__ out, unsigned long long *fx, unsigned *s_vmax,
+                                        unsigned *s_vmin, int *s_bad, unsigned *__restrict__ wide_count,
+                                        unsigned *__restrict__ wide_list, unsigned key)
 {
     float *fl = reinterpret_cast<float *>(fx);
     int x0, y0, z0;
     tile_origin(it.tile, T, x0, y0, z0);
     if (threadIdx.x == 0) {
-        *s_cmax = 0u;
+        *s_vmax = 0u;
+        *s_vmin = 0x7f800000u;   // +inf: no nonzero value yet
         *s_bad = 0;
     }
     for (int i = threadIdx.x; i < TPAD; i += TILE_THREADS) fx[i] = 0ull;
     const int end = it.start + it.count;
     float vv[RECS_PER_THREAD];
-    float vmax = 0.0f;
+    float vmax = 0.0f, vmin = INFINITY;
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < RECS_PER_THREAD; ++k) {
@@ -735,22 +838,29 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         if (r < end) {
             const float v = event_value(values, y, p, kind, eps, rec_ev[r]);
             vv[k] = v;
-            if (isfinite(v)) vmax = fmaxf(vmax, fabsf(v));
-            else bad = true;
+            if (isfinite(v)) {
+                vmax = fmaxf(vmax, fabsf(v));
+                if (v != 0.0f) vmin = fminf(vmin, fabsf(v));
+            } else {
+                bad = true;
+            }
         }
     }
-    const float c = vmax * (adjoint ? it.wmax_fp : it.wmax_bp);
-    const unsigned m = isfinite(c) ? __float_as_uint(c) : 0x7f7fffffu;
     __syncthreads();
-    if (m) atomicMax(s_cmax, m);
+    // magnitudes of finite non-negative floats order as their bit patterns
+    if (vmax > 0.0f) atomicMax(s_vmax, __float_as_uint(vmax));
+    if (vmin < INFINITY) atomicMin(s_vmin, __float_as_uint(vmin));
     if (bad) *s_bad = 1;
     __syncthreads();
     const bool fixed = *s_bad == 0;
-    int shift = 0;
-    if (fixed && *s_cmax) {
-        int ex = 0;
-        (void)frexpf(__uint_as_float(*s_cmax), &ex);   // cmax < 2^ex
-        shift = FX_BITS - ex;
+    const float vtop = __uint_as_float(*s_vmax);
+    int nb = 1, etop = 0, shift = 0;
+    if (fixed && vtop > 0.0f) {
+        int ebot = 0;
+        (void)frexpf(vtop, &etop);                       // vtop < 2^etop
+        (void)frexpf(__uint_as_float(*s_vmin), &ebot);
+        nb = min((etop - ebot) / BAND_BITS + 1, MAX_BANDS);
+        shift = band_shift(vtop, adjoint ? it.wmax_fp : it.wmax_bp, etop, 0);
     }
     // the next record is loaded while this one is marched
     SegRec Rn;
@@ -762,6 +872,8 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         if (r >= end) continue;
         const SegRec R = Rn;
         if (r + TILE_THREADS < end) Rn = recs[r + TILE_THREADS];
+        // a wide-range item marches band 0 here, its other bands in k_bp_bands
+        if (nb > 1 && (v == 0.0f || value_band(v, etop, nb) != 0)) continue;
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
         const int a = (int)(ab & 0xffffu), b = (int)(ab >> 16);
         if (a >= b) continue;
@@ -779,6 +891,8 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
     } else {
         for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = fl[tile_word(i)];
     }
+    // list a wide-range item for k_bp_bands
+    if (nb > 1 && threadIdx.x == 0) wide_list[atomicAdd(wide_count, 1u)] = key;
 }
 
 template <bool TOF, bool DS, bool DZ>
@@ -789,15 +903,39 @@ __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_tiled(const Se
                                                            const float *__restrict__ y, const float *__restrict__ p,
                                                            int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
                                                            const int *__restrict__ rec_ev,
-                                                           float *__restrict__ tile_out)
+                                                           float *__restrict__ tile_out,
+                                                           unsigned *__restrict__ wide)
 {
     __shared__ unsigned long long fx[TPAD + TILE_THREADS];   // tile + one dummy slot per lane
-    __shared__ unsigned s_cmax;
+    __shared__ unsigned s_vmax, s_vmin;
     __shared__ int s_bad;
     const int k = order[blockIdx.x];
     const Item it = items[k];
     bp_item<TOF, DS, DZ>(recs, it, values, y, p, kind, eps, adjoint, G, T, rec_ev, tile_out + (int64_t)k * TVOX, fx,
-                         &s_cmax, &s_bad);
+                         &s_vmax, &s_vmin, &s_bad, wide, wide + 1, (unsigned)k);
+}
+
+// The further bands of the items k_bp_tiled listed in wide (wide[0] items,
+// wide[1..]): a small grid that leaves at once when the list is empty.
+template <bool TOF, bool DS, bool DZ>
+__global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_bands(const SegRec *__restrict__ recs,
+                                                           const Item *__restrict__ items,
+                                                           const float *__restrict__ values,
+                                                           const float *__restrict__ y, const float *__restrict__ p,
+                                                           int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
+                                                           const int *__restrict__ rec_ev,
+                                                           float *__restrict__ tile_out,
+                                                           const unsigned *__restrict__ wide)
+{
+    __shared__ unsigned long long fx[TPAD + TILE_THREADS];
+    __shared__ unsigned s_vmax, s_vmin;
+    const unsigned n = wide[0];
+    for (unsigned w = blockIdx.x; w < n; w += gridDim.x) {
+        const int k = (int)wide[1 + w];
+        const Item it = items[k];
+        bp_item_bands<TOF, DS, DZ>(recs, it, values, y, p, kind, eps, adjoint, G, T, rec_ev,
+                                   tile_out + (int64_t)k * TVOX, fx, &s_vmax, &s_vmin);
+    }
 }
 
 template <bool TOF, bool DS, bool DZ>
@@ -1072,10 +1210,12 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fp_batch(const ScanDesc *__res
 template <bool TOF, bool DS, bool DZ>
 __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_batch(const ScanDesc *__restrict__ desc,
                                                            const Work *__restrict__ work, int nwork,
-                                                           unsigned *__restrict__ counter, int persistent)
+                                                           unsigned *__restrict__ counter, int persistent,
+                                                           unsigned *__restrict__ wide_count,
+                                                           unsigned *__restrict__ wide_list)
 {
     __shared__ unsigned long long fx[TPAD + TILE_THREADS];   // tile + one dummy slot per lane
-    __shared__ unsigned s_cmax;
+    __shared__ unsigned s_vmax, s_vmin;
     __shared__ int s_bad;
     __shared__ int s_next;
     for (int w = blockIdx.x; w < nwork; w = next_work(persistent, counter, &s_next)) {
@@ -1085,7 +1225,29 @@ __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_batch(const Sc
         const LaunchGeom G = d->G;
         const TileCfg T = d->T;
         bp_item<TOF, DS, DZ>(d->recs, it, d->q, nullptr, nullptr, 0, 0.0f, 0, G, T, d->rec_ev,
-                             d->tile_buf + (int64_t)wk.item * TVOX, fx, &s_cmax, &s_bad);
+                             d->tile_buf + (int64_t)wk.item * TVOX, fx, &s_vmax, &s_vmin, &s_bad, wide_count,
+                             wide_list, (unsigned)w);
+    }
+}
+
+// k_bp_bands over a batch group's listed work items (indices into `work`).
+template <bool TOF, bool DS, bool DZ>
+__global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_bands_batch(const ScanDesc *__restrict__ desc,
+                                                                 const Work *__restrict__ work,
+                                                                 const unsigned *__restrict__ wide_count,
+                                                                 const unsigned *__restrict__ wide_list)
+{
+    __shared__ unsigned long long fx[TPAD + TILE_THREADS];
+    __shared__ unsigned s_vmax, s_vmin;
+    const unsigned n = *wide_count;
+    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
+        const Work wk = work[wide_list[i]];
+        const ScanDesc *d = desc + wk.scan;
+        const Item it = d->items[wk.item];
+        const LaunchGeom G = d->G;
+        const TileCfg T = d->T;
+        bp_item_bands<TOF, DS, DZ>(d->recs, it, d->q, nullptr, nullptr, 0, 0.0f, 0, G, T, d->rec_ev,
+                                   d->tile_buf + (int64_t)wk.item * TVOX, fx, &s_vmax, &s_vmin);
     }
 }
 
@@ -1179,6 +1341,7 @@ struct srf_plan {
     DevBuf<uint8_t> ev_flag;
     DevBuf<float> partial;
     DevBuf<float> tile_buf;
+    DevBuf<unsigned> wide;   // [0]: items listed for k_bp_bands, [1..]: their indices
     // binning scratch: freed after srf_plan_create, kept by srf_plan_rebuild
     DevBuf<int> ev_cnt;
     DevBuf<unsigned> key_count;
@@ -1313,6 +1476,7 @@ void plan_free(srf_plan *P)
     dfree(P->ev_flag, s);
     dfree(P->partial, s);
     dfree(P->tile_buf, s);
+    dfree(P->wide, s);
     free_scratch(P, s);
     delete P;
 }
@@ -1321,7 +1485,8 @@ size_t plan_bytes(const srf_plan *P)
 {
     return P->recs.cap * sizeof(SegRec) + P->rec_ev.cap * sizeof(int) + P->items.cap * sizeof(Item) +
            P->tile_items.cap * sizeof(int) + P->order.cap * sizeof(int) + P->ev_off.cap * sizeof(int) +
-           P->ev_flag.cap + P->partial.cap * sizeof(float) + P->tile_buf.cap * sizeof(float);
+           P->ev_flag.cap + P->partial.cap * sizeof(float) + P->tile_buf.cap * sizeof(float) +
+           P->wide.cap * sizeof(unsigned);
 }
 
 }  // namespace
@@ -1455,6 +1620,7 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     PB_TRY(ensure(P->tile_items, tile_items.size(), s));
     PB_TRY(ensure(P->order, items.size(), s));
     PB_TRY(ensure(P->tile_buf, (size_t)std::max(1, nitems) * TVOX, s));
+    PB_TRY(ensure(P->wide, (size_t)nitems + 1, s));
     if (!items.empty())
         PB_HIP(hipMemcpyAsync(P->items.p, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
     PB_HIP(hipMemcpyAsync(P->tile_items.p, tile_items.data(), tile_items.size() * sizeof(int),
@@ -1691,12 +1857,15 @@ static int plan_bp(srf_plan *P, const float *values, const float *y, const float
         }
         return launch_status();
     }
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, P->wide.p, 0u, (int64_t)1);
     mark_before(P, s);
-    if (P->nitems > 0)
-        PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values, y, p, kind, eps,
-                                                                       (int)(mode == SRF_BP_ADJOINT), P->G, P->T,
-                                                                       P->rec_ev.p, P->tile_buf.p));
+    PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values, y, p,
+                                                                   kind, eps, (int)(mode == SRF_BP_ADJOINT), P->G, P->T,
+                                                                   P->rec_ev.p, P->tile_buf.p, P->wide.p));
     mark_after(P, s);
+    PLAN_DISPATCH(k_bp_bands, <<<std::min(P->nitems, BANDS_GRID), TILE_THREADS, 0, s>>>(
+                                  P->recs.p, P->items.p, values, y, p, kind, eps, (int)(mode == SRF_BP_ADJOINT),
+                                  P->G, P->T, P->rec_ev.p, P->tile_buf.p, P->wide.p));
     dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
     hipLaunchKernelGGL(k_tile_reduce, grid, dim3(256), 0, s, P->tile_buf.p, P->tile_items.p, P->G, P->T, accumulate,
                        image);
@@ -1739,12 +1908,15 @@ int srf_plan_backproject_update(srf_plan *P, const float *values, float *x, cons
                            nv);
         return launch_status();
     }
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, P->wide.p, 0u, (int64_t)1);
     mark_before(P, s);
-    if (P->nitems > 0)
-        PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values,
-                                                                       nullptr, nullptr, 0, 0.0f, 0, P->G, P->T,
-                                                                       P->rec_ev.p, P->tile_buf.p));
+    PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values,
+                                                                   nullptr, nullptr, 0, 0.0f, 0, P->G, P->T,
+                                                                   P->rec_ev.p, P->tile_buf.p, P->wide.p));
     mark_after(P, s);
+    PLAN_DISPATCH(k_bp_bands, <<<std::min(P->nitems, BANDS_GRID), TILE_THREADS, 0, s>>>(
+                                  P->recs.p, P->items.p, values, nullptr, nullptr, 0, 0.0f, 0, P->G, P->T,
+                                  P->rec_ev.p, P->tile_buf.p, P->wide.p));
     dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
     hipLaunchKernelGGL(k_tile_update, grid, dim3(256), 0, s, P->tile_buf.p, P->tile_items.p, P->G, P->T, emap,
                        zero_guard, x);
@@ -1772,7 +1944,9 @@ struct srf_batch {
     int nwork;
     int64_t *ev_base;
     int *tile_base;
-    unsigned *counter;   // [2 g]: projector queue of group g, [2 g + 1]: backprojector queue
+    unsigned *counter;   // [2 g]: projector queue of group g, [2 g + 1]: backprojector queue,
+                         // [2 NVARIANT + g]: group g's items listed for k_bp_bands_batch
+    unsigned *wide;      // listed work indices; group g's at wide + g_start[g]
     int64_t total_ev;
     int total_tiles;
     int persistent;
@@ -1789,6 +1963,7 @@ static void batch_free(srf_batch *B)
     (void)hipFree(B->ev_base);
     (void)hipFree(B->tile_base);
     (void)hipFree(B->counter);
+    (void)hipFree(B->wide);
     delete B;
 }
 
@@ -1902,7 +2077,8 @@ int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int3
     SRF_TRY(dmalloc(&B->work, std::max<size_t>(1, work.size()), B->bytes));
     SRF_TRY(dmalloc(&B->ev_base, ev_base.size(), B->bytes));
     SRF_TRY(dmalloc(&B->tile_base, tile_base.size(), B->bytes));
-    SRF_TRY(dmalloc(&B->counter, 2 * NVARIANT, B->bytes));
+    SRF_TRY(dmalloc(&B->counter, 3 * NVARIANT, B->bytes));
+    SRF_TRY(dmalloc(&B->wide, std::max<size_t>(1, work.size()), B->bytes));
     SRF_HIP(hipMemcpyAsync(B->desc, desc.data(), desc.size() * sizeof(ScanDesc), hipMemcpyHostToDevice, s));
     if (!work.empty())
         SRF_HIP(hipMemcpyAsync(B->work, work.data(), work.size() * sizeof(Work), hipMemcpyHostToDevice, s));
@@ -1955,13 +2131,16 @@ int srf_batch_backproject_update(srf_batch *B, float zero_guard, void *stream)
 {
     if (!B) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
-    if (B->persistent && B->nwork > 0)
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, B->counter, 0u, (int64_t)(2 * NVARIANT));
+    if (B->nwork > 0)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, B->counter, 0u, (int64_t)(3 * NVARIANT));
     for (int g = 0; g < NVARIANT; ++g) {
         if (B->g_count[g] == 0) continue;
+        unsigned *wc = B->counter + 2 * NVARIANT + g, *wl = B->wide + B->g_start[g];
         BATCH_DISPATCH(g, k_bp_batch, <<<B->g_grid[g], TILE_THREADS, 0, s>>>(B->desc, B->work + B->g_start[g],
                                                                              B->g_count[g], B->counter + 2 * g + 1,
-                                                                             B->persistent));
+                                                                             B->persistent, wc, wl));
+        BATCH_DISPATCH(g, k_bp_bands_batch, <<<std::min(B->g_count[g], BANDS_GRID), TILE_THREADS, 0, s>>>(
+                                                B->desc, B->work + B->g_start[g], wc, wl));
     }
     if (B->total_tiles > 0)
         hipLaunchKernelGGL(k_tile_update_batch, dim3((unsigned)B->total_tiles * (TVOX / 256)), dim3(256), 0, s,

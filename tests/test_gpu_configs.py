@@ -9,8 +9,10 @@
 - the batched workload at its full size (32 scans, events log-uniform in
   1e5..2e7, in-plane 96..256, 64..128 slices, 2..4 mm voxels; 8 subsets x 2
   iterations, no scatter): every scan of the batched HIP-graph run
-  bit-identical to its per-scan run, and the largest scan's OSEM on a
-  2e5-event subsample against the oracle.
+  bit-identical to its per-scan run, and the largest scan's OSEM against the
+  oracle on a subsample of 2e5 events per subset (1.6e6 events: with fewer,
+  most of its ~1e6 voxels get no term in a subset, fall to 0 after the first
+  sub-iteration, and the EM ratio then amplifies float rounding).
 
 Bars: 1e-5 relative L2 for projections / backprojections, 1e-4 for EM
 images, bitwise for batched vs per-scan."""
@@ -101,7 +103,7 @@ def test_batched_workload_full_size(orc):
     assert all(same)
     # the largest scan's OSEM on a 2e5-event subsample against the oracle
     big = max(scans, key=lambda s: int(s.lors.shape[0]))
-    ns = 200_000
+    ns = 8 * 200_000
     idx = scan_permutation(int(big.lors.shape[0]), 99, big.lors.device)[:ns].sort().values
     lors = big.lors[idx].contiguous()
     vals = big.meta["inv_att"][idx].contiguous()
@@ -115,5 +117,5 @@ def test_batched_workload_full_size(orc):
     want = orc.osem(vals.cpu().numpy(), lors.cpu().numpy(), emap, list(big.center), list(big.size), 16, 8,
                     perm.cpu().numpy(), tof=(TOF_BIN, s2), driver="osem", guard_emap=True)
     err = rel_l2(x, want, emap > 0)
-    print(f"largest scan {tuple(big.shape)}, 2e5-event subsample, 2 iterations vs oracle: {err:.3g}", flush=True)
+    print(f"largest scan {tuple(big.shape)}, {ns}-event subsample, 2 iterations vs oracle: {err:.3g}", flush=True)
     assert err <= 1e-4

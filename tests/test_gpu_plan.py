@@ -294,3 +294,116 @@ def test_full_subset_size_properties():
     print(f"<Ax,y> {lhs:.10g} <x,A^T y> {rhs:.10g}", flush=True)
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
     plan.close()
+
+
+@pytest.mark.parametrize("shape,voxel", [((96, 96, 106), 2.8), ((40, 36, 13), 3.0), ((33, 64, 9), 2.0),
+                                         ((64, 64, 64), 2.5)],
+                         ids=["96x96x106", "40x36x13", "33x64x9", "64x64x64"])
+@pytest.mark.parametrize("window", [0.0, 22.0])
+def test_plan_on_grids_not_aligned_to_tiles(orc, shape, voxel, window):
+    """Tiled plans on grids whose sizes are not multiples of the 32 x 32 x 8
+    tile (partial tiles at the high edges): the plan's digest equals the event
+    kernels' (window 0) or holds a subset of their terms (window 22), and its
+    projection and backprojection are within 1e-5 of the oracle's."""
+    import torch
+    from srf_amd import ops, phantom
+    act, mu, center, size = phantom.random_ellipsoid_phantom(shape, voxel, 21, device="cuda")
+    s2 = phantom.tof_sigma2_from_fwhm(60.0)
+    lors = phantom.sample_events(act, center, size, 40000, 424.0, 300.0, 22, tof_fwhm_mm=60.0, device="cuda")
+    geom = list(center) + list(size)
+    plan = ops.Plan(lors, list(shape), geom, True, 3.66, s2, tof_window=window)
+    for mode in (1, 0):
+        h0, c0 = ops.lor_digest(lors, list(shape), geom, True, 3.66, s2, mode)
+        h1, c1 = plan.digest(mode)
+        if window == 0.0:
+            bad = int((h0 != h1).sum() + (c0 != c1).sum())
+            print(f"{shape} mode {mode}: {bad} events differ from the event kernels", flush=True)
+            assert bad == 0
+        else:
+            assert bool((c1 <= c0).all())
+    ln = lors.cpu().numpy()
+    a = act.cpu().numpy()
+    p = plan.project(ops.to_kernel_layout(act.contiguous())).cpu().numpy()
+    p_o = orc.project(a, ln, center, size, tof=(3.66, s2), f64=True)
+    y = np.random.default_rng(3).uniform(0.5, 1.5, len(ln)).astype(np.float32)
+    bp = ops.from_kernel_layout(plan.backproject(torch.from_numpy(y).cuda(), 0)).cpu().numpy()
+    bp_o = orc.backproject(y, ln, shape, center, size, tof=(3.66, s2))
+    ep, eb = rel_l2(p, p_o), rel_l2(bp, bp_o)
+    print(f"{shape} x {voxel} mm, window {window}: projection {ep:.3g}, backprojection {eb:.3g}", flush=True)
+    assert ep <= 1e-5 and eb <= 1e-5
+    plan.close()
+
+
+def test_tof_window_keeps_events_centred_outside_the_image():
+    """Events whose TOF centre lies 10-80 mm outside a 128 mm field of view
+    (their in-image terms are all Gaussian tail) mixed with ordinary ones:
+    the EM ratio y / sum(terms) spreads each of them over its tail, so a plan
+    with TOF window 22 must keep that tail (the window is relative to the
+    largest in-image factor).  One EM sub-iteration with window 22 equals
+    the exact plan's to 1e-6."""
+    import torch
+    from srf_amd import ops
+    g = torch.Generator().manual_seed(31)
+    n = 60000
+    shape, geom = [64, 64, 32], [0.0, 0.0, 0.0, 128.0, 128.0, 64.0]
+    yz = (torch.rand(n, 2, generator=g) - 0.5) * torch.tensor([100.0, 50.0])
+    slope = (torch.rand(n, 2, generator=g) - 0.5) * 0.3
+    p1 = torch.stack([torch.full((n,), -300.0), yz[:, 0] - 300 * slope[:, 0], yz[:, 1] - 300 * slope[:, 1]], 1)
+    p2 = torch.stack([torch.full((n,), 300.0), yz[:, 0] + 300 * slope[:, 0], yz[:, 1] + 300 * slope[:, 1]], 1)
+    # TOF centre s mm from the midpoint towards p2 (tof = -2 s): half inside
+    # the field of view, half 10-80 mm beyond its edge on either side
+    s = (torch.rand(n, generator=g) - 0.5) * 100.0
+    out = torch.rand(n, generator=g) < 0.5
+    side = torch.where(torch.rand(n, generator=g) < 0.5, -1.0, 1.0)
+    s = torch.where(out, side * (64.0 + 10.0 + 70.0 * torch.rand(n, generator=g)), s)
+    lors = torch.cat([p1, p2, (-2.0 * s)[:, None]], 1).float().cuda().contiguous()
+    exact = ops.Plan(lors, shape, geom, True, 3.66, 649.2)
+    win = ops.Plan(lors, shape, geom, True, 3.66, 649.2, tof_window=22.0)
+    x0 = torch.rand((32, 64, 64), device="cuda", generator=torch.Generator("cuda").manual_seed(2)) + 0.5
+    emap = torch.ones_like(x0)
+    y = torch.ones(n, device="cuda")
+    res = []
+    for plan in (exact, win):
+        x = x0.clone()
+        q = plan.project_ratio(x, y, 0, 1e-8)
+        plan.backproject_update(q, x, emap, 0.0)
+        res.append(x.cpu().numpy())
+    _, c0 = exact.digest(0)
+    _, c1 = win.digest(0)
+    err = rel_l2(res[1], res[0])
+    print(f"terms kept {float(c1.double().sum() / c0.double().sum()):.3f}; one sub-iteration, window 22 vs "
+          f"exact: {err:.3g}", flush=True)
+    assert err <= 1e-6
+    exact.close()
+    win.close()
+
+
+def test_backprojection_keeps_small_values_beside_huge_ones(orc):
+    """Work items whose event values span 15 decades (the EM ratio of an event
+    with a projection ~0 is 1e8 times the others'): 1% of the events carry 1e9,
+    the rest ~1e-6.  One fixed-point scale per item resolves 2^-50 of the
+    largest term, ~1e-6 here, and would round the small events' terms away;
+    with exponent bands every voxel whose value comes at least half from the
+    small events is within 1e-5 of the oracle's float64 sum, relative to that
+    voxel's own value."""
+    import torch
+    from siddon_ref import random_cylinder_lors
+    from srf_amd import ops
+    lors = random_cylinder_lors(40000, 150.0, 80.0, 23, tof_sigma=25.0)
+    shape, center, size = (48, 48, 24), [0.0, 0.0, 0.0], [96.0, 96.0, 48.0]
+    rng = np.random.default_rng(4)
+    v = rng.uniform(0.5e-6, 1.5e-6, len(lors)).astype(np.float32)
+    v[rng.random(len(lors)) < 0.01] = 1e9
+    plan = ops.Plan(torch.from_numpy(lors).cuda(), list(shape), center + size, True, 3.66, 649.2)
+    small_v = np.where(v > 1.0, 0, v).astype(np.float32)
+    for mode in (0, 1):
+        var = orc.BP if mode == 0 else orc.FP
+        bp = ops.from_kernel_layout(plan.backproject(torch.from_numpy(v).cuda(), mode)).cpu().numpy()
+        want = orc.backproject(v, lors, shape, center, size, tof=(3.66, 649.2), variant=var)
+        small = orc.backproject(small_v, lors, shape, center, size, tof=(3.66, 649.2), variant=var)
+        m = (want > 0) & (small >= 0.5 * want)
+        rel = np.abs(bp[m] - want[m]) / want[m]
+        print(f"mode {mode}: {int(m.sum())} voxels mostly from the 1e-6 events, max relative error {rel.max():.3g}; "
+              f"relative L2 over all {rel_l2(bp, want):.3g}", flush=True)
+        assert m.sum() > 1000 and rel.max() <= 1e-5 and rel_l2(bp, want) <= 1e-6
+    plan.close()

@@ -152,3 +152,39 @@ def test_fixed_point_term_is_rounded_product():
     t = np.array([2.5, 3.5, -2.5, -0.5, 0.5, -1.0], np.float64)
     got_t = (t + 6755399441055744.0).view(np.int64) - np.int64(0x4338000000000000)
     assert got_t.tolist() == [2, 4, -2, 0, 0, -1]
+
+
+def test_backprojection_value_bands():
+    """The backprojector's exponent bands (srf_tiled.hip, bp_item /
+    bp_item_bands): with vtop < 2^etop the item's largest value, a nonzero
+    value v belongs to band min((etop - e_v) // 16, nb - 1) (e_v: |v| < 2^e_v)
+    and band b's shift is 49 - ex(c_b), c_0 = vtop * W, c_b = 2^(etop - 16 b) W.
+    Emulated on wide random items: every value lands in one band, no term of a
+    band reaches 2^49 after its shift (no overflow), and outside the last band
+    every value keeps at least 2^32 of resolution (a band spans 2^16)."""
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        n = int(rng.integers(1, 8193))
+        v = (rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-30, 30, n)).astype(np.float32)
+        v[rng.random(n) < 0.05] = 0.0
+        wmax = np.float32(10.0 ** rng.uniform(-3, 2))
+        nz = v[v != 0]
+        if nz.size == 0:
+            continue
+        a = np.abs(nz).astype(np.float64)
+        vtop, vbot = a.max(), a.min()
+        etop, ebot = np.frexp(np.float32(vtop))[1], np.frexp(np.float32(vbot))[1]
+        nb = min((etop - ebot) // 16 + 1, 8)
+        ev = np.frexp(nz)[1]
+        band = np.minimum((etop - ev) // 16, nb - 1)
+        assert band.min() >= 0 and band.max() <= nb - 1
+        for b in range(nb):
+            c = np.float32(vtop) * wmax if b == 0 else np.ldexp(wmax, int(etop - 16 * b))
+            shift = 49 - int(np.frexp(np.float32(c))[1])
+            vb = a[band == b]
+            if vb.size == 0:
+                continue
+            scaled = np.ldexp(vb * np.float64(wmax), shift)
+            assert scaled.max() < 2.0 ** 49
+            if b < nb - 1:
+                assert scaled.min() >= 2.0 ** 32
