@@ -131,6 +131,18 @@ int srf_lor_digest(const float *lors, int64_t n, int64_t lor_stride,
  * (no counterpart in the reference). */
 int srf_smooth(const float *x, int32_t n0, int32_t n1, int32_t n2, float sigma0, float sigma1, float sigma2,
                const float *mask, float *tmp, float *out, void *stream);
+/* srf_smooth without the tmp argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The intermediate image
+ * lives in a static device array of 2^23 floats (32 MiB) that the library
+ * holds on every device it runs on until the process ends; n0*n1*n2 above 2^23
+ * (a 256 x 256 x 128 image) returns SRF_ERR_WORKSPACE.  Nothing is allocated
+ * and nothing needs clearing between calls.  All calls on one device share
+ * that array: calls that overlap on one device (several streams or host
+ * threads) race and must be serialised by the caller.  Same kernels and the
+ * same results as srf_smooth, bit for bit. */
+int srf_smooth_noalloc(const float *x, int32_t n0, int32_t n1, int32_t n2, float sigma0, float sigma1,
+                       float sigma2, const float *mask, float *out, void *stream);
 
 /* In-plane PSF blur of x (nx ny, nz; slices fastest) with the sparse K_xy of
  * srfnef's Deconvolute (corrections/psf/deconvolute.py:56-62, built by

@@ -83,6 +83,7 @@ SIGNATURES = [
     ("srf_em_update", ctypes.c_int, [_P, _P, _P, _I64, _F, _P]),
     ("srf_lor_digest", ctypes.c_int, [_P, _I64, _I64, _P, _I64, SrfGrid, _TOFP, ctypes.c_int, _P, _P, _P]),
     ("srf_smooth", ctypes.c_int, [_P, _I32, _I32, _I32, _F, _F, _F, _P, _P, _P, _P]),
+    ("srf_smooth_noalloc", ctypes.c_int, [_P, _I32, _I32, _I32, _F, _F, _F, _P, _P, _P]),
     ("srf_pair_lors", ctypes.c_int, [_P, _I64, _I64, _I64, _P, _P]),
     ("srf_psf_xy", ctypes.c_int, [_P, _I32, _I32, _I32, _P, _I32, _I32, _P, _F, _P, _P]),
     ("srf_crystal_index", ctypes.c_int, [_P, _I64, _I64, _I32, ctypes.POINTER(SrfEcat), ctypes.POINTER(SrfCyl), _P,

@@ -7,10 +7,20 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "srf_hip.h"
 #include "srf_internal.hpp"
 
 namespace srf {
+
+// Scratch of srf_smooth_noalloc: the intermediate image of the separable
+// passes, 2^23 floats (32 MiB, a 256 x 256 x 128 image) in a file-scope device
+// array, so that entry point allocates nothing.  Each device holds its own copy
+// for the life of the process.  In host code `scratch` only names the symbol:
+// its device address, per device, comes from hipGetSymbolAddress.
+constexpr int64_t SMOOTH_SCRATCH_FLOATS = (int64_t)1 << 23;
+static __device__ __align__(256) float scratch[SMOOTH_SCRATCH_FLOATS];
 
 __global__ __launch_bounds__(256) void k_fill(float *__restrict__ dst, float v, int64_t n)
 {
@@ -230,6 +240,47 @@ int srf_smooth(const float *x, int32_t n0, int32_t n1, int32_t n2, float sigma0,
     hipLaunchKernelGGL(k_gauss_pass, dim3(nb), dim3(256), 0, s, (const float *)tmp, n, n0, (int64_t)n1 * n2,
                        sigma0, rad(sigma0), mask, out);
     return launch_status();
+}
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mu;
+float *scratch_dev[SCRATCH_DEVICES];   // device address of `scratch` on each device, once resolved
+}  // namespace
+
+// The current device's address of `scratch`, resolved by hipGetSymbolAddress on
+// the first call on that device (with it current) and kept.
+static int scratch_address(float **ptr)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_DEVICES) {
+        (void)hipGetLastError();
+        return SRF_ERR_LAUNCH;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return SRF_ERR_LAUNCH;
+        }
+        scratch_dev[dev] = static_cast<float *>(p);
+    }
+    *ptr = scratch_dev[dev];
+    return SRF_OK;
+}
+
+int srf_smooth_noalloc(const float *x, int32_t n0, int32_t n1, int32_t n2, float sigma0, float sigma1, float sigma2,
+                       const float *mask, float *out, void *stream)
+{
+    if (!x || !out || n0 <= 0 || n1 <= 0 || n2 <= 0) return SRF_ERR_INVALID_ARG;
+    if ((int64_t)n0 * n1 * n2 > SMOOTH_SCRATCH_FLOATS) return SRF_ERR_WORKSPACE;
+    float *tmp = nullptr;
+    const int rc = scratch_address(&tmp);
+    if (rc) return rc;
+    // every scratch word a call reads is written by the same call first (the
+    // second pass fills it whole before the third reads it): nothing to clear
+    return srf_smooth(x, n0, n1, n2, sigma0, sigma1, sigma2, mask, tmp, out, stream);
 }
 
 int srf_image_to_kernel_layout(const float *src_xyz, float *dst_kernel, int32_t nx, int32_t ny, int32_t nz,
