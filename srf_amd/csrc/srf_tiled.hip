@@ -715,16 +715,17 @@ __device__ __forceinline__ float event_value(const float *__restrict__ values, c
 //
 // One scale resolves a term to 2^-50 of the item's largest; an item whose
 // event values span more than 2^BAND_BITS (the EM ratio y / (p + eps) of an
-// event whose projection is ~0 is 1e8 times the others') would lose its small
-// terms.  Its records are then accumulated in exponent bands: band b holds the
-// values in [2^(e - 16 (b + 1)), 2^(e - 16 b)), e the exponent of the largest,
-// each band with its own scale in its own pass over the records, and the
-// bands' float tiles are summed in band order (still bit-reproducible).  An
-// item within one band -- every item of ordinary data -- takes the single pass.
+// event whose projection is ~0 can be 1e8 times the others') would lose its
+// small terms.  Its records are then accumulated in exponent bands: band b
+// holds the values in [2^(e - 24 (b + 1)), 2^(e - 24 b)), e the exponent of
+// the largest, each band with its own scale in its own pass over the records
+// (bp_item_bands), and the bands' float tiles are summed in band order (still
+// bit-reproducible).  Within a band a value resolves to 2^-25 of itself or
+// better, float32's own precision.  An item within one band -- the ordinary
+// case -- takes the single pass.
 constexpr int FX_BITS = 49;
-constexpr int BAND_BITS = 16;
-constexpr int BANDS_GRID = 256;   // workgroups of k_bp_bands (wide-range items are rare)
-constexpr int MAX_BANDS = 8;   // values below 2^-128 of the largest join the last band
+constexpr int BAND_BITS = 24;
+constexpr int MAX_BANDS = 8;   // values below 2^-192 of the largest join the last band
 constexpr int RECS_PER_THREAD = ITEM / TILE_THREADS;
 static_assert(ITEM % TILE_THREADS == 0, "whole records per thread");
 
@@ -747,46 +748,31 @@ __device__ __forceinline__ int band_shift(float vtop, float wmax, int etop, int 
     return FX_BITS - ex;
 }
 
-// Bands 1 .. nb-1 of a wide-range item, added to the band-0 tile in `out`
-// (k_bp_bands, for the items k_bp_tiled listed).  The item's values are read
-// again and reduced again in s_vmax / s_vmin.
+// All bands of a wide-range item (largest value vtop < 2^etop, nb bands), each
+// in its own pass over the records: band 0 written to `out`, the others added.
+// Only such items run this; it is kept compact (records not unrolled, values
+// read again) so that it needs fewer registers than bp_item's single pass.
 template <bool TOF, bool DS, bool DZ>
 __device__ __forceinline__ void bp_item_bands(const SegRec *__restrict__ recs, const Item &it,
                                               const float *__restrict__ values, const float *__restrict__ y,
                                               const float *__restrict__ p, int kind, float eps, int adjoint,
                                               const LaunchGeom &G, const TileCfg &T,
                                               const int *__restrict__ rec_ev, float *__restrict__ out,
-                                              unsigned long long *fx, unsigned *s_vmax, unsigned *s_vmin)
+                                              unsigned long long *fx, float vtop, int etop, int nb)
 {
     float *fl = reinterpret_cast<float *>(fx);
     int x0, y0, z0;
     tile_origin(it.tile, T, x0, y0, z0);
     const float wmax = adjoint ? it.wmax_fp : it.wmax_bp;
     const int end = it.start + it.count;
-    __syncthreads();   // the previous item's tile and reductions are done with
-    if (threadIdx.x == 0) {
-        *s_vmax = 0u;
-        *s_vmin = 0x7f800000u;
-    }
-    __syncthreads();
-    float vmax = 0.0f, vmin = INFINITY;
-    for (int r = it.start + threadIdx.x; r < end; r += TILE_THREADS) {
-        const float v = fabsf(event_value(values, y, p, kind, eps, rec_ev[r]));
-        vmax = fmaxf(vmax, v);
-        if (v != 0.0f) vmin = fminf(vmin, v);
-    }
-    if (vmax > 0.0f) atomicMax(s_vmax, __float_as_uint(vmax));
-    if (vmin < INFINITY) atomicMin(s_vmin, __float_as_uint(vmin));
-    __syncthreads();
-    const float vtop = __uint_as_float(*s_vmax);
-    int etop = 0, ebot = 0;
-    (void)frexpf(vtop, &etop);
-    (void)frexpf(__uint_as_float(*s_vmin), &ebot);
-    const int nb = min((etop - ebot) / BAND_BITS + 1, MAX_BANDS);
-    for (int band = 1; band < nb; ++band) {
+    for (int band = 0; band < nb; ++band) {
         const int shift = band_shift(vtop, wmax, etop, band);
-        for (int i = threadIdx.x; i < TPAD; i += TILE_THREADS) fx[i] = 0ull;
-        __syncthreads();
+        if (band > 0) {
+            __syncthreads();   // the previous band's tile has been written out
+            for (int i = threadIdx.x; i < TPAD; i += TILE_THREADS) fx[i] = 0ull;
+            __syncthreads();
+        }
+#pragma unroll 1
         for (int r = it.start + threadIdx.x; r < end; r += TILE_THREADS) {
             const float v = event_value(values, y, p, kind, eps, rec_ev[r]);
             if (v == 0.0f || value_band(v, etop, nb) != band) continue;
@@ -801,9 +787,10 @@ __device__ __forceinline__ void bp_item_bands(const SegRec *__restrict__ recs, c
             march_any<TOF, DS, DZ, M_SCATTER_FX>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS)
-            out[i] = out[i] + ldexpf((float)(long long)fx[tile_word(i)], -shift);
-        __syncthreads();
+        for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) {
+            const float bv = ldexpf((float)(long long)fx[tile_word(i)], -shift);
+            out[i] = band == 0 ? bv : out[i] + bv;
+        }
     }
 }
 
@@ -815,8 +802,7 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
                                         const float *__restrict__ p, int kind, float eps, int adjoint,
                                         const LaunchGeom &G, const TileCfg &T, const int *__restrict__ rec_ev,
                                         float *__restrict__ out, unsigned long long *fx, unsigned *s_vmax,
-                                        unsigned *s_vmin, int *s_bad, unsigned *__restrict__ wide_count,
-                                        unsigned *__restrict__ wide_list, unsigned key)
+                                        unsigned *s_vmin, int *s_bad)
 {
     float *fl = reinterpret_cast<float *>(fx);
     int x0, y0, z0;
@@ -862,6 +848,11 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         nb = min((etop - ebot) / BAND_BITS + 1, MAX_BANDS);
         shift = band_shift(vtop, adjoint ? it.wmax_fp : it.wmax_bp, etop, 0);
     }
+    if (nb > 1) {   // wide-range item (the branch is uniform over the workgroup)
+        bp_item_bands<TOF, DS, DZ>(recs, it, values, y, p, kind, eps, adjoint, G, T, rec_ev, out, fx, vtop, etop,
+                                   nb);
+        return;
+    }
     // the next record is loaded while this one is marched
     SegRec Rn;
     if (it.start + (int)threadIdx.x < end) Rn = recs[it.start + threadIdx.x];
@@ -872,8 +863,6 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         if (r >= end) continue;
         const SegRec R = Rn;
         if (r + TILE_THREADS < end) Rn = recs[r + TILE_THREADS];
-        // a wide-range item marches band 0 here, its other bands in k_bp_bands
-        if (nb > 1 && (v == 0.0f || value_band(v, etop, nb) != 0)) continue;
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
         const int a = (int)(ab & 0xffffu), b = (int)(ab >> 16);
         if (a >= b) continue;
@@ -891,8 +880,6 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
     } else {
         for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) out[i] = fl[tile_word(i)];
     }
-    // list a wide-range item for k_bp_bands
-    if (nb > 1 && threadIdx.x == 0) wide_list[atomicAdd(wide_count, 1u)] = key;
 }
 
 template <bool TOF, bool DS, bool DZ>
@@ -903,8 +890,7 @@ __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_tiled(const Se
                                                            const float *__restrict__ y, const float *__restrict__ p,
                                                            int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
                                                            const int *__restrict__ rec_ev,
-                                                           float *__restrict__ tile_out,
-                                                           unsigned *__restrict__ wide)
+                                                           float *__restrict__ tile_out)
 {
     __shared__ unsigned long long fx[TPAD + TILE_THREADS];   // tile + one dummy slot per lane
     __shared__ unsigned s_vmax, s_vmin;
@@ -912,30 +898,7 @@ __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_tiled(const Se
     const int k = order[blockIdx.x];
     const Item it = items[k];
     bp_item<TOF, DS, DZ>(recs, it, values, y, p, kind, eps, adjoint, G, T, rec_ev, tile_out + (int64_t)k * TVOX, fx,
-                         &s_vmax, &s_vmin, &s_bad, wide, wide + 1, (unsigned)k);
-}
-
-// The further bands of the items k_bp_tiled listed in wide (wide[0] items,
-// wide[1..]): a small grid that leaves at once when the list is empty.
-template <bool TOF, bool DS, bool DZ>
-__global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_bands(const SegRec *__restrict__ recs,
-                                                           const Item *__restrict__ items,
-                                                           const float *__restrict__ values,
-                                                           const float *__restrict__ y, const float *__restrict__ p,
-                                                           int kind, float eps, int adjoint, LaunchGeom G, TileCfg T,
-                                                           const int *__restrict__ rec_ev,
-                                                           float *__restrict__ tile_out,
-                                                           const unsigned *__restrict__ wide)
-{
-    __shared__ unsigned long long fx[TPAD + TILE_THREADS];
-    __shared__ unsigned s_vmax, s_vmin;
-    const unsigned n = wide[0];
-    for (unsigned w = blockIdx.x; w < n; w += gridDim.x) {
-        const int k = (int)wide[1 + w];
-        const Item it = items[k];
-        bp_item_bands<TOF, DS, DZ>(recs, it, values, y, p, kind, eps, adjoint, G, T, rec_ev,
-                                   tile_out + (int64_t)k * TVOX, fx, &s_vmax, &s_vmin);
-    }
+                         &s_vmax, &s_vmin, &s_bad);
 }
 
 template <bool TOF, bool DS, bool DZ>
@@ -1210,9 +1173,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fp_batch(const ScanDesc *__res
 template <bool TOF, bool DS, bool DZ>
 __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_batch(const ScanDesc *__restrict__ desc,
                                                            const Work *__restrict__ work, int nwork,
-                                                           unsigned *__restrict__ counter, int persistent,
-                                                           unsigned *__restrict__ wide_count,
-                                                           unsigned *__restrict__ wide_list)
+                                                           unsigned *__restrict__ counter, int persistent)
 {
     __shared__ unsigned long long fx[TPAD + TILE_THREADS];   // tile + one dummy slot per lane
     __shared__ unsigned s_vmax, s_vmin;
@@ -1225,29 +1186,7 @@ __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_batch(const Sc
         const LaunchGeom G = d->G;
         const TileCfg T = d->T;
         bp_item<TOF, DS, DZ>(d->recs, it, d->q, nullptr, nullptr, 0, 0.0f, 0, G, T, d->rec_ev,
-                             d->tile_buf + (int64_t)wk.item * TVOX, fx, &s_vmax, &s_vmin, &s_bad, wide_count,
-                             wide_list, (unsigned)w);
-    }
-}
-
-// k_bp_bands over a batch group's listed work items (indices into `work`).
-template <bool TOF, bool DS, bool DZ>
-__global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_bands_batch(const ScanDesc *__restrict__ desc,
-                                                                 const Work *__restrict__ work,
-                                                                 const unsigned *__restrict__ wide_count,
-                                                                 const unsigned *__restrict__ wide_list)
-{
-    __shared__ unsigned long long fx[TPAD + TILE_THREADS];
-    __shared__ unsigned s_vmax, s_vmin;
-    const unsigned n = *wide_count;
-    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
-        const Work wk = work[wide_list[i]];
-        const ScanDesc *d = desc + wk.scan;
-        const Item it = d->items[wk.item];
-        const LaunchGeom G = d->G;
-        const TileCfg T = d->T;
-        bp_item_bands<TOF, DS, DZ>(d->recs, it, d->q, nullptr, nullptr, 0, 0.0f, 0, G, T, d->rec_ev,
-                                   d->tile_buf + (int64_t)wk.item * TVOX, fx, &s_vmax, &s_vmin);
+                             d->tile_buf + (int64_t)wk.item * TVOX, fx, &s_vmax, &s_vmin, &s_bad);
     }
 }
 
@@ -1341,7 +1280,6 @@ struct srf_plan {
     DevBuf<uint8_t> ev_flag;
     DevBuf<float> partial;
     DevBuf<float> tile_buf;
-    DevBuf<unsigned> wide;   // [0]: items listed for k_bp_bands, [1..]: their indices
     // binning scratch: freed after srf_plan_create, kept by srf_plan_rebuild
     DevBuf<int> ev_cnt;
     DevBuf<unsigned> key_count;
@@ -1476,7 +1414,6 @@ void plan_free(srf_plan *P)
     dfree(P->ev_flag, s);
     dfree(P->partial, s);
     dfree(P->tile_buf, s);
-    dfree(P->wide, s);
     free_scratch(P, s);
     delete P;
 }
@@ -1485,8 +1422,7 @@ size_t plan_bytes(const srf_plan *P)
 {
     return P->recs.cap * sizeof(SegRec) + P->rec_ev.cap * sizeof(int) + P->items.cap * sizeof(Item) +
            P->tile_items.cap * sizeof(int) + P->order.cap * sizeof(int) + P->ev_off.cap * sizeof(int) +
-           P->ev_flag.cap + P->partial.cap * sizeof(float) + P->tile_buf.cap * sizeof(float) +
-           P->wide.cap * sizeof(unsigned);
+           P->ev_flag.cap + P->partial.cap * sizeof(float) + P->tile_buf.cap * sizeof(float);
 }
 
 }  // namespace
@@ -1620,7 +1556,6 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     PB_TRY(ensure(P->tile_items, tile_items.size(), s));
     PB_TRY(ensure(P->order, items.size(), s));
     PB_TRY(ensure(P->tile_buf, (size_t)std::max(1, nitems) * TVOX, s));
-    PB_TRY(ensure(P->wide, (size_t)nitems + 1, s));
     if (!items.empty())
         PB_HIP(hipMemcpyAsync(P->items.p, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
     PB_HIP(hipMemcpyAsync(P->tile_items.p, tile_items.data(), tile_items.size() * sizeof(int),
@@ -1857,15 +1792,11 @@ static int plan_bp(srf_plan *P, const float *values, const float *y, const float
         }
         return launch_status();
     }
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, P->wide.p, 0u, (int64_t)1);
     mark_before(P, s);
     PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values, y, p,
                                                                    kind, eps, (int)(mode == SRF_BP_ADJOINT), P->G, P->T,
-                                                                   P->rec_ev.p, P->tile_buf.p, P->wide.p));
+                                                                   P->rec_ev.p, P->tile_buf.p));
     mark_after(P, s);
-    PLAN_DISPATCH(k_bp_bands, <<<std::min(P->nitems, BANDS_GRID), TILE_THREADS, 0, s>>>(
-                                  P->recs.p, P->items.p, values, y, p, kind, eps, (int)(mode == SRF_BP_ADJOINT),
-                                  P->G, P->T, P->rec_ev.p, P->tile_buf.p, P->wide.p));
     dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
     hipLaunchKernelGGL(k_tile_reduce, grid, dim3(256), 0, s, P->tile_buf.p, P->tile_items.p, P->G, P->T, accumulate,
                        image);
@@ -1908,15 +1839,11 @@ int srf_plan_backproject_update(srf_plan *P, const float *values, float *x, cons
                            nv);
         return launch_status();
     }
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, P->wide.p, 0u, (int64_t)1);
     mark_before(P, s);
     PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values,
                                                                    nullptr, nullptr, 0, 0.0f, 0, P->G, P->T,
-                                                                   P->rec_ev.p, P->tile_buf.p, P->wide.p));
+                                                                   P->rec_ev.p, P->tile_buf.p));
     mark_after(P, s);
-    PLAN_DISPATCH(k_bp_bands, <<<std::min(P->nitems, BANDS_GRID), TILE_THREADS, 0, s>>>(
-                                  P->recs.p, P->items.p, values, nullptr, nullptr, 0, 0.0f, 0, P->G, P->T,
-                                  P->rec_ev.p, P->tile_buf.p, P->wide.p));
     dim3 grid((TVOX + 255) / 256, (unsigned)P->T.ntiles);
     hipLaunchKernelGGL(k_tile_update, grid, dim3(256), 0, s, P->tile_buf.p, P->tile_items.p, P->G, P->T, emap,
                        zero_guard, x);
@@ -1944,9 +1871,7 @@ struct srf_batch {
     int nwork;
     int64_t *ev_base;
     int *tile_base;
-    unsigned *counter;   // [2 g]: projector queue of group g, [2 g + 1]: backprojector queue,
-                         // [2 NVARIANT + g]: group g's items listed for k_bp_bands_batch
-    unsigned *wide;      // listed work indices; group g's at wide + g_start[g]
+    unsigned *counter;   // [2 g]: projector queue of group g, [2 g + 1]: backprojector queue
     int64_t total_ev;
     int total_tiles;
     int persistent;
@@ -1963,7 +1888,6 @@ static void batch_free(srf_batch *B)
     (void)hipFree(B->ev_base);
     (void)hipFree(B->tile_base);
     (void)hipFree(B->counter);
-    (void)hipFree(B->wide);
     delete B;
 }
 
@@ -2077,8 +2001,7 @@ int srf_batch_create(srf_plan *const *plans, const srf_batch_scan_t *scans, int3
     SRF_TRY(dmalloc(&B->work, std::max<size_t>(1, work.size()), B->bytes));
     SRF_TRY(dmalloc(&B->ev_base, ev_base.size(), B->bytes));
     SRF_TRY(dmalloc(&B->tile_base, tile_base.size(), B->bytes));
-    SRF_TRY(dmalloc(&B->counter, 3 * NVARIANT, B->bytes));
-    SRF_TRY(dmalloc(&B->wide, std::max<size_t>(1, work.size()), B->bytes));
+    SRF_TRY(dmalloc(&B->counter, 2 * NVARIANT, B->bytes));
     SRF_HIP(hipMemcpyAsync(B->desc, desc.data(), desc.size() * sizeof(ScanDesc), hipMemcpyHostToDevice, s));
     if (!work.empty())
         SRF_HIP(hipMemcpyAsync(B->work, work.data(), work.size() * sizeof(Work), hipMemcpyHostToDevice, s));
@@ -2131,16 +2054,13 @@ int srf_batch_backproject_update(srf_batch *B, float zero_guard, void *stream)
 {
     if (!B) return SRF_ERR_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
-    if (B->nwork > 0)
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, B->counter, 0u, (int64_t)(3 * NVARIANT));
+    if (B->persistent && B->nwork > 0)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, B->counter, 0u, (int64_t)(2 * NVARIANT));
     for (int g = 0; g < NVARIANT; ++g) {
         if (B->g_count[g] == 0) continue;
-        unsigned *wc = B->counter + 2 * NVARIANT + g, *wl = B->wide + B->g_start[g];
         BATCH_DISPATCH(g, k_bp_batch, <<<B->g_grid[g], TILE_THREADS, 0, s>>>(B->desc, B->work + B->g_start[g],
                                                                              B->g_count[g], B->counter + 2 * g + 1,
-                                                                             B->persistent, wc, wl));
-        BATCH_DISPATCH(g, k_bp_bands_batch, <<<std::min(B->g_count[g], BANDS_GRID), TILE_THREADS, 0, s>>>(
-                                                B->desc, B->work + B->g_start[g], wc, wl));
+                                                                             B->persistent));
     }
     if (B->total_tiles > 0)
         hipLaunchKernelGGL(k_tile_update_batch, dim3((unsigned)B->total_tiles * (TVOX / 256)), dim3(256), 0, s,

@@ -157,11 +157,11 @@ def test_fixed_point_term_is_rounded_product():
 def test_backprojection_value_bands():
     """The backprojector's exponent bands (srf_tiled.hip, bp_item /
     bp_item_bands): with vtop < 2^etop the item's largest value, a nonzero
-    value v belongs to band min((etop - e_v) // 16, nb - 1) (e_v: |v| < 2^e_v)
-    and band b's shift is 49 - ex(c_b), c_0 = vtop * W, c_b = 2^(etop - 16 b) W.
+    value v belongs to band min((etop - e_v) // 24, nb - 1) (e_v: |v| < 2^e_v)
+    and band b's shift is 49 - ex(c_b), c_0 = vtop * W, c_b = 2^(etop - 24 b) W.
     Emulated on wide random items: every value lands in one band, no term of a
     band reaches 2^49 after its shift (no overflow), and outside the last band
-    every value keeps at least 2^32 of resolution (a band spans 2^16)."""
+    every value keeps at least 2^24 of resolution (a band spans 2^24)."""
     rng = np.random.default_rng(11)
     for _ in range(200):
         n = int(rng.integers(1, 8193))
@@ -174,12 +174,12 @@ def test_backprojection_value_bands():
         a = np.abs(nz).astype(np.float64)
         vtop, vbot = a.max(), a.min()
         etop, ebot = np.frexp(np.float32(vtop))[1], np.frexp(np.float32(vbot))[1]
-        nb = min((etop - ebot) // 16 + 1, 8)
+        nb = min((etop - ebot) // 24 + 1, 8)
         ev = np.frexp(nz)[1]
-        band = np.minimum((etop - ev) // 16, nb - 1)
+        band = np.minimum((etop - ev) // 24, nb - 1)
         assert band.min() >= 0 and band.max() <= nb - 1
         for b in range(nb):
-            c = np.float32(vtop) * wmax if b == 0 else np.ldexp(wmax, int(etop - 16 * b))
+            c = np.float32(vtop) * wmax if b == 0 else np.ldexp(wmax, int(etop - 24 * b))
             shift = 49 - int(np.frexp(np.float32(c))[1])
             vb = a[band == b]
             if vb.size == 0:
@@ -187,4 +187,4 @@ def test_backprojection_value_bands():
             scaled = np.ldexp(vb * np.float64(wmax), shift)
             assert scaled.max() < 2.0 ** 49
             if b < nb - 1:
-                assert scaled.min() >= 2.0 ** 32
+                assert scaled.min() >= 2.0 ** 24
