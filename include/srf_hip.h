@@ -347,13 +347,23 @@ int srf_scatter_correct(const float *y, const float *weight, const int32_t *bins
 
 /* y[v][out_off + n] = act(bias[n] + sum_t sum_c x[v + o_t][c] w[t][n][c]) for
  * n < cout; act = leaky ReLU of `slope` (slope < 0: identity).  Writes y_bf16
- * and/or y_f32 (row stride out_stride).  bias may be NULL. */
+ * and/or y_f32 (row stride out_stride).  bias may be NULL.  x and w 16-byte
+ * aligned.  The _batch form convolves N samples of D x H x W stacked along z
+ * (N*D slices; taps never reach across a sample's z faces). */
 int srf_cnn_conv(const void *x, const void *w, const float *bias, int32_t D, int32_t H, int32_t W,
                  int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, int32_t out_stride, int32_t out_off,
                  float slope, void *y_bf16, float *y_f32, void *stream);
-/* dw[t][n][c] += sum_v dy[v][n] x[v + o_t][c]   (dy bf16 rows of dy_stride) */
+int srf_cnn_conv_batch(const void *x, const void *w, const float *bias, int32_t N, int32_t D, int32_t H, int32_t W,
+                       int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, int32_t out_stride, int32_t out_off,
+                       float slope, void *y_bf16, float *y_f32, void *stream);
+/* dw[t][n][c] += sum_v dy[v][n] x[v + o_t][c]   (dy bf16 rows of dy_stride);
+ * the _batch form over N stacked samples, and with db != NULL also
+ * db[n] += sum_v dy[v][n] (the bias gradient, as srf_cnn_bias_grad). */
 int srf_cnn_conv_wgrad(const void *x, const void *dy, int32_t dy_stride, int32_t D, int32_t H, int32_t W,
                        int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, float *dw, void *stream);
+int srf_cnn_conv_wgrad_batch(const void *x, const void *dy, int32_t dy_stride, int32_t N, int32_t D, int32_t H,
+                             int32_t W, int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, float *dw,
+                             float *db, void *stream);
 /* wf[t][c][n] = w[taps-1-t][n][c]: the kernel whose srf_cnn_conv of dy is the
  * gradient with respect to x (wf is [taps][fcout_p][fcin_p]). */
 int srf_cnn_flip_weights(const void *w, int32_t taps, int32_t cout, int32_t cout_p, int32_t cin_p, int32_t cin,

@@ -30,6 +30,8 @@ namespace srf {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
@@ -61,59 +63,209 @@ __device__ __forceinline__ void tap_offset(int t, int taps, int &dz, int &dy, in
     dx = t % 3 - 1;
 }
 
-// One 32 x 32 output block per wavefront, four wavefronts per workgroup along v.
-__global__ __launch_bounds__(256) void k_conv(const unsigned short *__restrict__ X,
-                                              const unsigned short *__restrict__ Wt,
-                                              const float *__restrict__ bias, ConvDims d,
-                                              unsigned short *__restrict__ Ybf, float *__restrict__ Yf)
+// Convolution as an implicit GEMM on gfx950's v_mfma_f32_16x16x32_bf16
+// (A: 16 voxels x 32 K values, B: 32 K values x 16 output channels, FP32
+// accumulation).  K runs over (tap, group of 8 input channels).
+//
+// A workgroup of four wavefronts owns a tile of up to 128 voxels (tx x ty x tz,
+// x fastest; wavefront w rows 32w..32w+31, two 16-row blocks) and NBW blocks of
+// 16 output channels.  Per chunk of 8, 16 or 32 input channels the tile's input
+// halo is staged in LDS with coalesced 16-byte loads: zero outside the volume
+// and outside the tile's own sample (a batch stacks samples along z, Ds slices
+// each; a tile never straddles two), so taps never read a neighbouring
+// sample.  A lane then reads its 8 K values of a voxel as one 16-byte LDS
+// access; a halo voxel's channel groups are padded to an odd count of 16-byte
+// words so that the 16 voxels of a row block start in different banks.
+// Weights (a few KiB to 440 KiB per layer) are read from global memory
+// through the caches.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct ConvTile {
+    int tx, ty, tz;    // tile extent
+    int ntx, nty;      // tiles along x and y
+    int Ds;            // slices per sample (tz divides it)
+};
+constexpr int CONV_TILE = 128;
+constexpr int CONV_LDS_MAX = 65536;
+
+// Offset in halo voxels of tap t from the halo voxel of the tap (-1,-1,-1) corner.
+__device__ __forceinline__ int tap_delta(int t, int taps, int hx, int hyx)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int V = d.D * d.H * d.W;
-    const int v0 = (blockIdx.x * 4 + wave) * 32;
-    const int n0 = blockIdx.y * 32;
-    if (v0 >= V) return;
-    const int m = lane & 31;              // A row / B column within the block
-    const int kq = (lane >> 5) * 4;       // first of the lane's 4 k values
-    const int v = v0 + m;
-    const bool vin = v < V;
-    const int x = vin ? v % d.W : 0, y = vin ? (v / d.W) % d.H : 0, z = vin ? v / (d.W * d.H) : 0;
-    f32x16 acc = {0};
-    for (int t = 0; t < d.taps; ++t) {
-        int dz, dy, dx;
-        tap_offset(t, d.taps, dz, dy, dx);
-        const int zz = z + dz, yy = y + dy, xx = x + dx;
-        const bool ok = vin && zz >= 0 && zz < d.D && yy >= 0 && yy < d.H && xx >= 0 && xx < d.W;
-        const int64_t src = ((int64_t)(zz * d.H + yy) * d.W + xx) * d.cin_p;
-        const unsigned short *wrow = Wt + ((int64_t)t * d.cout_p + n0 + m) * d.cin_p;
-        for (int c = 0; c < d.cin_p; c += 8) {
-            bf16x4 a = {0, 0, 0, 0};
-            if (ok) a = *reinterpret_cast<const bf16x4 *>(X + src + c + kq);
-            const bf16x4 b = *reinterpret_cast<const bf16x4 *>(wrow + c + kq);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-        }
-    }
-    // C layout: lane l holds column n = l % 32, rows 8*(i/4) + 4*(l/32) + i%4
-    const int n = n0 + m;
-    if (n >= d.cout) return;
-    const float bn = bias ? bias[n] : 0.0f;
+    if (taps == 1) return 0;
+    return (t / 9) * hyx + ((t / 3) % 3) * hx + t % 3;
+}
+
+template <int TAPS, int NG, int NBW>
+__device__ __forceinline__ void conv_chunk(const u16x8 *__restrict__ halo, const int (&hb)[2], int gp, int hx, int hyx,
+                                           const unsigned short *__restrict__ wl, int64_t tap_stride, int nb_stride,
+                                           int q, f32x4 (&acc)[2][NBW])
+{
+    constexpr int G = TAPS * NG;          // K groups of 8 in this chunk
+    constexpr int S = (G + 3) / 4;        // MFMA steps (4 groups each)
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = v0 + 8 * (i / 4) + kq + (i % 4);
-        if (row >= V) continue;
-        float r = acc[i] + bn;
-        if (d.slope >= 0.0f && r < 0.0f) r *= d.slope;
-        const int64_t o = (int64_t)row * d.out_stride + d.out_off + n;
-        if (Ybf) Ybf[o] = f2bf(r);
-        if (Yf) Yf[o] = r;
+    for (int s = 0; s < S; ++s) {
+        int t, g;
+        if constexpr (NG == 4) {
+            t = s;
+            g = q;
+        } else if constexpr (NG == 2) {
+            t = 2 * s + (q >> 1);
+            g = q & 1;
+        } else {
+            t = 4 * s + q;
+            g = 0;
+        }
+        // tap offsets: uniform candidates, one picked per lane
+        int delta;
+        if constexpr (NG == 4) {
+            delta = tap_delta(s, TAPS, hx, hyx);
+        } else if constexpr (NG == 2) {
+            const int d0 = tap_delta(2 * s, TAPS, hx, hyx), d1 = tap_delta(2 * s + 1, TAPS, hx, hyx);
+            delta = (q >> 1) ? d1 : d0;
+        } else {
+            const int d0 = tap_delta(4 * s, TAPS, hx, hyx), d1 = tap_delta(4 * s + 1, TAPS, hx, hyx);
+            const int d2 = tap_delta(4 * s + 2, TAPS, hx, hyx), d3 = tap_delta(4 * s + 3, TAPS, hx, hyx);
+            delta = q == 0 ? d0 : q == 1 ? d1 : q == 2 ? d2 : d3;
+        }
+        const bool valid = (G % 4 == 0 || s < S - 1) || (4 * s + q < G);
+        u16x8 a[2], w[NBW];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) a[r] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) w[j] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        if (valid) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) a[r] = halo[(hb[r] + delta) * gp + g];
+#pragma unroll
+            for (int j = 0; j < NBW; ++j)
+                w[j] = *reinterpret_cast<const u16x8 *>(wl + t * tap_stride + j * nb_stride + 8 * g);
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int j = 0; j < NBW; ++j)
+                acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[r]),
+                                                                    __builtin_bit_cast(bf16x8, w[j]), acc[r][j], 0,
+                                                                    0, 0);
     }
 }
 
+template <int TAPS, int NBW>
+__global__ __launch_bounds__(256) void k_conv(const unsigned short *__restrict__ X,
+                                              const unsigned short *__restrict__ Wt,
+                                              const float *__restrict__ bias, ConvDims d, ConvTile T,
+                                              unsigned short *__restrict__ Ybf, float *__restrict__ Yf)
+{
+    constexpr int R = TAPS == 27 ? 1 : 0;
+    extern __shared__ u16x8 halo[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, q = lane >> 4;
+    int b = blockIdx.x;
+    const int bx = b % T.ntx;
+    b /= T.ntx;
+    const int by = b % T.nty;
+    const int bz = b / T.nty;
+    const int x0 = bx * T.tx, y0 = by * T.ty, z0 = bz * T.tz;
+    const int zs0 = (z0 / T.Ds) * T.Ds;   // first slice of the tile's sample
+    const int hx = T.tx + 2 * R, hy = T.ty + 2 * R, hz = T.tz + 2 * R;
+    const int hyx = hx * hy, HV = hyx * hz;
+    const int nvox = T.tx * T.ty * T.tz;
+    const int n0 = blockIdx.y * 16 * NBW;
+    // this lane's A rows (clamped into the tile; rows past it are not stored)
+    int hb[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int tv = min(wave * 32 + r * 16 + col, nvox - 1);
+        const int ltx = tv % T.tx, lty = (tv / T.tx) % T.ty, ltz = tv / (T.tx * T.ty);
+        hb[r] = (ltz * hy + lty) * hx + ltx;
+    }
+    f32x4 acc[2][NBW];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) acc[r][j] = f32x4{0, 0, 0, 0};
+    const int64_t tap_stride = (int64_t)d.cout_p * d.cin_p;
+    const int nb_stride = 16 * d.cin_p;
+    for (int c0 = 0; c0 < d.cin_p;) {
+        const int left = (d.cin_p - c0) >> 3;
+        const int lg = left >= 4 ? 2 : left >= 2 ? 1 : 0;   // log2 of the chunk's 8-channel groups
+        const int ng = 1 << lg, gp = ng | 1;
+        __syncthreads();   // the previous chunk's reads are done
+        for (int i = threadIdx.x; i < (HV << lg); i += 256) {
+            const int g = i & (ng - 1), hv = i >> lg;
+            const int gx = x0 + hv % hx - R, gy = y0 + (hv / hx) % hy - R, gz = z0 + hv / hyx - R;
+            u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= zs0 && gz < zs0 + T.Ds)
+                val = *reinterpret_cast<const u16x8 *>(X + ((int64_t)(gz * d.H + gy) * d.W + gx) * d.cin_p + c0 +
+                                                       8 * g);
+            halo[hv * gp + g] = val;
+        }
+        __syncthreads();
+        const unsigned short *wl = Wt + (int64_t)(n0 + col) * d.cin_p + c0;
+        if (lg == 2)
+            conv_chunk<TAPS, 4, NBW>(halo, hb, gp, hx, hyx, wl, tap_stride, nb_stride, q, acc);
+        else if (lg == 1)
+            conv_chunk<TAPS, 2, NBW>(halo, hb, gp, hx, hyx, wl, tap_stride, nb_stride, q, acc);
+        else
+            conv_chunk<TAPS, 1, NBW>(halo, hb, gp, hx, hyx, wl, tap_stride, nb_stride, q, acc);
+        c0 += 8 * ng;
+    }
+    // C layout of a 16 x 16 block: lane l holds column l % 16, rows 4 * (l / 16) + i
+#pragma unroll
+    for (int j = 0; j < NBW; ++j) {
+        const int n = n0 + 16 * j + col;
+        if (n >= d.cout) continue;
+        const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int tv = wave * 32 + r * 16 + 4 * q + i;
+                if (tv >= nvox) continue;
+                const int gx = x0 + tv % T.tx, gy = y0 + (tv / T.tx) % T.ty, gz = z0 + tv / (T.tx * T.ty);
+                if (gx >= d.W || gy >= d.H) continue;
+                float v = acc[r][j][i] + bn;
+                if (d.slope >= 0.0f && v < 0.0f) v *= d.slope;
+                const int64_t o = ((int64_t)(gz * d.H + gy) * d.W + gx) * d.out_stride + d.out_off + n;
+                if (Ybf) Ybf[o] = f2bf(v);
+                if (Yf) Yf[o] = v;
+            }
+    }
+}
+
+// Tile of k_conv for samples of Ds x H x W: x first (up to 32), then y, then z
+// (a divisor of Ds), at most 128 voxels, and a halo that fits CONV_LDS_MAX
+// with 5 padded 16-byte words per voxel.
+static void conv_tile(int Ds, int H, int W, int R, ConvTile &T)
+{
+    T.tx = W < 32 ? W : 32;
+    T.ty = H < CONV_TILE / T.tx ? H : CONV_TILE / T.tx;
+    T.tz = 1;
+    for (int z = CONV_TILE / (T.tx * T.ty); z > 1; --z)
+        if (Ds % z == 0) {
+            T.tz = z;
+            break;
+        }
+    auto bytes = [&] { return (T.tx + 2 * R) * (T.ty + 2 * R) * (T.tz + 2 * R) * 5 * 16; };
+    while (bytes() > CONV_LDS_MAX && T.tz > 1) {
+        int z = T.tz - 1;
+        while (Ds % z) --z;
+        T.tz = z;
+    }
+    while (bytes() > CONV_LDS_MAX && T.ty > 1) --T.ty;
+    T.ntx = (W + T.tx - 1) / T.tx;
+    T.nty = (H + T.ty - 1) / T.ty;
+    T.Ds = Ds;
+}
+
 // Weight gradient of one tap: dW[t][co][ci] += sum_v dY[v][co] * X[v + o_t][ci]
-// (dY rows of stride dy_stride, channels [0, cout); X rows of stride cin_p).
-// Grid: (voxel chunks, taps, cout blocks of 32 * cin blocks of 32).
+// (dY rows of stride dy_stride, channels [0, cout); X rows of stride cin_p;
+// samples of Ds slices stacked along z).  The general-shape path: element
+// loads from global memory.  Grid: (voxel chunks, taps, cout blocks of 32 *
+// cin blocks of 32).
 __global__ __launch_bounds__(256) void k_conv_wgrad(const unsigned short *__restrict__ X,
                                                     const unsigned short *__restrict__ dY, int dy_stride,
-                                                    ConvDims d, int chunk, float *__restrict__ dW)
+                                                    ConvDims d, int Ds, int chunk, float *__restrict__ dW)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int V = d.D * d.H * d.W;
@@ -136,8 +288,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const unsigned short *__rest
             if (v < vend) {
                 if (co < d.cout) av = dY[(int64_t)v * dy_stride + co];
                 const int x = v % d.W, y = (v / d.W) % d.H, z = v / (d.W * d.H);
-                const int zz = z + dz, yy = y + dy, xx = x + dx;
-                if (ci < d.cin_p && zz >= 0 && zz < d.D && yy >= 0 && yy < d.H && xx >= 0 && xx < d.W)
+                const int zz = z + dz, yy = y + dy, xx = x + dx, zs = z - z % Ds;
+                if (ci < d.cin_p && zz >= zs && zz < zs + Ds && yy >= 0 && yy < d.H && xx >= 0 && xx < d.W)
                     bv = X[((int64_t)(zz * d.H + yy) * d.W + xx) * d.cin_p + ci];
             }
             a[j] = (short)av;
@@ -151,6 +303,167 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const unsigned short *__rest
         const int c = ci;                                   // column = input channel
         if (r < d.cout && c < d.cin_p && acc[i] != 0.0f)
             atomicAdd(dW + ((int64_t)t * d.cout_p + r) * d.cin_p + c, acc[i]);
+    }
+}
+
+// Weight gradient on gfx950's v_mfma_f32_16x16x32_bf16: dW[t][co][ci] +=
+// sum_v dY[v][co] X[v + o_t][ci], an M = cout, N = cin, K = voxels GEMM per
+// tap.  A workgroup owns a 32 x 32 (co, ci) block of all taps and a run of
+// voxel tiles (the forward kernel's tiles; tx a multiple of 8).  Per tile the
+// dY tile and the X halo are staged in LDS channel-major, so that a lane's 8 K
+// values (8 voxels along x) are one aligned 16-byte read of dY; for X the
+// tap's x shift (-1, 0, +1 voxel: 0, 2, 4 bytes past an aligned word) is a
+// funnel shift of an aligned 16-byte and a 4-byte read.  Wavefront w keeps
+// taps w, w + 4, ... for the whole run (27 taps) or K step w (1 tap) in
+// registers and adds them to dW once.  With db, the workgroups of the first
+// cin block also sum the staged dY tile into the bias gradient.
+constexpr int WG_DS = 136;   // dY tile row (128 voxels + 8: rows 272 bytes apart, 4 banks)
+
+__device__ __forceinline__ unsigned funnel2(unsigned hi, unsigned lo)
+{
+    return __builtin_amdgcn_alignbyte(hi, lo, 2);
+}
+
+template <int TAPS>
+__global__ __launch_bounds__(256) void k_conv_wgrad_lds(const unsigned short *__restrict__ X,
+                                                        const unsigned short *__restrict__ dY, int dy_stride,
+                                                        ConvDims d, ConvTile T, int ntiles, int run, int RW, int CS,
+                                                        float *__restrict__ dW, float *__restrict__ db)
+{
+    constexpr int R = TAPS == 27 ? 1 : 0;
+    constexpr int NT = TAPS == 27 ? 7 : 1;   // taps per wavefront
+    extern __shared__ unsigned short lds16[];
+    unsigned short *XT = lds16;              // [32 ci][hz][hy][RW]
+    unsigned short *DT = lds16 + 32 * CS;    // [32 co][WG_DS]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, q = lane >> 4;
+    const int co0 = blockIdx.y * 32, ci0 = blockIdx.z * 32;
+    const int hy = T.ty + 2 * R, hz = T.tz + 2 * R, hx = T.tx + 2 * R;
+    const int HV = hx * hy * hz;
+    const int nvox = T.tx * T.ty * T.tz;
+    const int ngx = min(4, (d.cin_p - ci0) >> 3);
+    // this lane's voxel groups (8 along x) of the four K steps
+    int vb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int tv = 32 * k + 8 * q;
+        vb[k] = 0;
+        if (tv < nvox) {
+            const int ltx = tv % T.tx, lty = (tv / T.tx) % T.ty, ltz = tv / (T.tx * T.ty);
+            vb[k] = (ltz * hy + lty) * RW + ltx;
+        }
+    }
+    f32x4 acc[NT][2][2];
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][r][j] = f32x4{0, 0, 0, 0};
+    float bsum = 0.0f;
+    const int t_end = min(ntiles, (int)(blockIdx.x + 1) * run);
+    for (int tile = blockIdx.x * run; tile < t_end; ++tile) {
+        const int bx = tile % T.ntx, by = (tile / T.ntx) % T.nty, bz = tile / (T.ntx * T.nty);
+        const int x0 = bx * T.tx, y0 = by * T.ty, z0 = bz * T.tz;
+        const int zs0 = (z0 / T.Ds) * T.Ds;
+        __syncthreads();   // the previous tile's reads are done
+        // X halo, transposed: 16-byte loads of 8 channels, 8 two-byte LDS stores
+        for (int i = threadIdx.x; i < HV * ngx; i += 256) {
+            const int g = i % ngx, hv = i / ngx;
+            const int hxi = hv % hx, hyi = (hv / hx) % hy, hzi = hv / (hx * hy);
+            const int gx = x0 + hxi - R, gy = y0 + hyi - R, gz = z0 + hzi - R;
+            u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= zs0 && gz < zs0 + T.Ds)
+                val = *reinterpret_cast<const u16x8 *>(X + ((int64_t)(gz * d.H + gy) * d.W + gx) * d.cin_p + ci0 +
+                                                       8 * g);
+            const int pos = (hzi * hy + hyi) * RW + hxi;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) XT[(8 * g + j) * CS + pos] = val[j];
+        }
+        // dY tile, transposed (zero outside the volume, past the tile and for co >= cout)
+        for (int i = threadIdx.x; i < 128 * 4; i += 256) {
+            const int g = i & 3, tv = i >> 2;
+            u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
+            const int gx = x0 + tv % T.tx, gy = y0 + (tv / T.tx) % T.ty, gz = z0 + tv / (T.tx * T.ty);
+            const int c = co0 + 8 * g;
+            if (tv < nvox && gx < d.W && gy < d.H && c < dy_stride) {
+                val = *reinterpret_cast<const u16x8 *>(dY + ((int64_t)(gz * d.H + gy) * d.W + gx) * dy_stride + c);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (c + j >= d.cout) val[j] = 0;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) DT[(8 * g + j) * WG_DS + tv] = val[j];
+        }
+        __syncthreads();
+        if (db && blockIdx.z == 0) {
+            const int c = threadIdx.x >> 3, part = threadIdx.x & 7;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) bsum += bf2f(DT[c * WG_DS + part * 16 + v]);
+        }
+        // A fragments (dY) of the four K steps, shared by every tap
+        u16x8 a[4][2];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+                a[k][r] = *reinterpret_cast<const u16x8 *>(DT + (r * 16 + col) * WG_DS + 32 * k + 8 * q);
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+            const int t = TAPS == 27 ? wave + 4 * i : 0;
+            if (t >= TAPS) continue;
+            int shift = 0, tdelta = 0;
+            if (TAPS == 27) {
+                shift = t % 3;   // dx + 1
+                tdelta = (t / 9) * hy * RW + ((t / 3) % 3) * RW;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (TAPS == 1 && k != wave) continue;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const unsigned short *p = XT + (j * 16 + col) * CS + vb[k] + tdelta;
+                    uint4 w = *reinterpret_cast<const uint4 *>(p);
+                    if (shift) {
+                        const unsigned nx = *reinterpret_cast<const unsigned *>(p + 8);
+                        if (shift == 2) {
+                            w = uint4{w.y, w.z, w.w, nx};
+                        } else {
+                            w = uint4{funnel2(w.y, w.x), funnel2(w.z, w.y), funnel2(w.w, w.z), funnel2(nx, w.w)};
+                        }
+                    }
+                    const bf16x8 b = __builtin_bit_cast(bf16x8, w);
+#pragma unroll
+                    for (int r = 0; r < 2; ++r)
+                        acc[i][r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[k][r]), b,
+                                                                               acc[i][r][j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // C layout: lane l holds column (ci) l % 16, rows (co) 4 * (l / 16) + e
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        const int t = TAPS == 27 ? wave + 4 * i : 0;
+        if (t >= TAPS) continue;
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int co = co0 + r * 16 + 4 * q + e, ci = ci0 + j * 16 + col;
+                    const float v = acc[i][r][j][e];
+                    if (co < d.cout && ci < d.cin_p && v != 0.0f)
+                        atomicAdd(dW + ((int64_t)t * d.cout_p + co) * d.cin_p + ci, v);
+                }
+    }
+    if (db && blockIdx.z == 0) {
+        bsum += __shfl_xor(bsum, 1, 8);
+        bsum += __shfl_xor(bsum, 2, 8);
+        bsum += __shfl_xor(bsum, 4, 8);
+        const int c = co0 + (threadIdx.x >> 3);
+        if ((threadIdx.x & 7) == 0 && c < d.cout && bsum != 0.0f) atomicAdd(db + c, bsum);
     }
 }
 
@@ -361,38 +674,109 @@ using namespace srf;
 
 extern "C" {
 
+int srf_cnn_conv_batch(const void *x, const void *w, const float *bias, int32_t N, int32_t D, int32_t H, int32_t W,
+                       int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, int32_t out_stride, int32_t out_off,
+                       float slope, void *y_bf16, float *y_f32, void *stream)
+{
+    if (!x || !w || N <= 0 || D <= 0 || H <= 0 || W <= 0) return SRF_ERR_INVALID_ARG;
+    if (cin_p <= 0 || cin_p % 8 || cout <= 0 || cout_p % 32 || cout > cout_p) return SRF_ERR_INVALID_ARG;
+    if (taps != 1 && taps != 27) return SRF_ERR_INVALID_ARG;
+    if (out_off < 0 || out_off + cout > out_stride || (!y_bf16 && !y_f32)) return SRF_ERR_INVALID_ARG;
+    if ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(w)) & 15) return SRF_ERR_INVALID_ARG;
+    if ((int64_t)N * D * H * W * (cin_p > out_stride ? cin_p : out_stride) >= ((int64_t)1 << 31))
+        return SRF_ERR_UNSUPPORTED;
+    ConvDims d{N * D, H, W, cin_p, cout, cout_p, out_stride, out_off, taps, slope};
+    const int R = taps == 27 ? 1 : 0;
+    ConvTile T;
+    conv_tile(D, H, W, R, T);
+    const int nb = (cout + 15) / 16;
+    const int nbw = (nb >= 4 && cout_p % 64 == 0) ? 4 : nb >= 2 ? 2 : 1;
+    const int lg = cin_p >= 32 ? 2 : cin_p >= 16 ? 1 : 0;
+    const size_t lds = (size_t)(T.tx + 2 * R) * (T.ty + 2 * R) * (T.tz + 2 * R) * ((1 << lg) | 1) * 16;
+    dim3 grid((unsigned)((int64_t)T.ntx * T.nty * (N * D / T.tz)), (unsigned)((nb + nbw - 1) / nbw));
+    auto *xs = (const unsigned short *)x;
+    auto *ws = (const unsigned short *)w;
+    auto *yb = (unsigned short *)y_bf16;
+    auto st = (hipStream_t)stream;
+#define SRF_CONV(TP, NBW) hipLaunchKernelGGL((k_conv<TP, NBW>), grid, dim3(256), lds, st, xs, ws, bias, d, T, yb, y_f32)
+    if (taps == 27) {
+        if (nbw == 4) SRF_CONV(27, 4);
+        else if (nbw == 2) SRF_CONV(27, 2);
+        else SRF_CONV(27, 1);
+    } else {
+        if (nbw == 4) SRF_CONV(1, 4);
+        else if (nbw == 2) SRF_CONV(1, 2);
+        else SRF_CONV(1, 1);
+    }
+#undef SRF_CONV
+    return launch_status();
+}
+
 int srf_cnn_conv(const void *x, const void *w, const float *bias, int32_t D, int32_t H, int32_t W, int32_t cin_p,
                  int32_t cout, int32_t cout_p, int32_t taps, int32_t out_stride, int32_t out_off, float slope,
                  void *y_bf16, float *y_f32, void *stream)
 {
-    if (!x || !w || D <= 0 || H <= 0 || W <= 0) return SRF_ERR_INVALID_ARG;
-    if (cin_p <= 0 || cin_p % 8 || cout <= 0 || cout_p % 32 || cout > cout_p) return SRF_ERR_INVALID_ARG;
+    return srf_cnn_conv_batch(x, w, bias, 1, D, H, W, cin_p, cout, cout_p, taps, out_stride, out_off, slope, y_bf16,
+                              y_f32, stream);
+}
+
+int srf_cnn_conv_wgrad_batch(const void *x, const void *dy, int32_t dy_stride, int32_t N, int32_t D, int32_t H,
+                             int32_t W, int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, float *dw,
+                             float *db, void *stream)
+{
+    if (!x || !dy || !dw || N <= 0 || D <= 0 || H <= 0 || W <= 0) return SRF_ERR_INVALID_ARG;
+    if (cin_p <= 0 || cin_p % 8 || cout <= 0 || cout_p % 32 || cout > cout_p || dy_stride < cout)
+        return SRF_ERR_INVALID_ARG;
     if (taps != 1 && taps != 27) return SRF_ERR_INVALID_ARG;
-    if (out_off < 0 || out_off + cout > out_stride || (!y_bf16 && !y_f32)) return SRF_ERR_INVALID_ARG;
-    if ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(w)) & 7) return SRF_ERR_INVALID_ARG;
-    ConvDims d{D, H, W, cin_p, cout, cout_p, out_stride, out_off, taps, slope};
-    const int64_t V = (int64_t)D * H * W;
-    dim3 grid((unsigned)((V + 127) / 128), (unsigned)(cout_p / 32));
-    hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, (hipStream_t)stream, (const unsigned short *)x,
-                       (const unsigned short *)w, bias, d, (unsigned short *)y_bf16, y_f32);
-    return launch_status();
+    if ((int64_t)N * D * H * W * (cin_p > dy_stride ? cin_p : dy_stride) >= ((int64_t)1 << 31))
+        return SRF_ERR_UNSUPPORTED;
+    ConvDims d{N * D, H, W, cin_p, cout, cout_p, 0, 0, taps, -1.0f};
+    auto st = (hipStream_t)stream;
+    const int R = taps == 27 ? 1 : 0;
+    ConvTile T;
+    conv_tile(D, H, W, R, T);
+    // the LDS path: x tiles of whole 8-voxel groups, 16-byte aligned rows
+    const int RW = (T.tx + 2 * R + 7) / 8 * 8;
+    int CS = 0;
+    bool lds_ok = T.tx % 8 == 0 && dy_stride % 8 == 0 &&
+                  !((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(dy)) & 15);
+    if (lds_ok) {
+        const int hv = RW * (T.ty + 2 * R) * (T.tz + 2 * R);
+        CS = (hv + 119) / 128 * 128 + 8;   // >= hv; channel rows 4 banks apart
+        lds_ok = (size_t)64 * (CS + WG_DS) <= CONV_LDS_MAX;
+    }
+    if (lds_ok) {
+        const int ntiles = T.ntx * T.nty * (N * D / T.tz);
+        const int nblk = (cout_p / 32) * ((cin_p + 31) / 32);
+        // about 1024 workgroups, each adding its taps to dW once per run of tiles
+        int run = (int)(((int64_t)ntiles * nblk + 1023) / 1024);
+        run = run < 1 ? 1 : run > 64 ? 64 : run;
+        dim3 grid((unsigned)((ntiles + run - 1) / run), (unsigned)(cout_p / 32), (unsigned)((cin_p + 31) / 32));
+        const size_t lds = (size_t)64 * (CS + WG_DS);
+        if (taps == 27)
+            hipLaunchKernelGGL(k_conv_wgrad_lds<27>, grid, dim3(256), lds, st, (const unsigned short *)x,
+                               (const unsigned short *)dy, dy_stride, d, T, ntiles, run, RW, CS, dw, db);
+        else
+            hipLaunchKernelGGL(k_conv_wgrad_lds<1>, grid, dim3(256), lds, st, (const unsigned short *)x,
+                               (const unsigned short *)dy, dy_stride, d, T, ntiles, run, RW, CS, dw, db);
+        return launch_status();
+    }
+    const int64_t V = (int64_t)N * D * H * W;
+    const int chunk = 4096;
+    dim3 grid((unsigned)((V + chunk - 1) / chunk), (unsigned)taps, (unsigned)((cout_p / 32) * ((cin_p + 31) / 32)));
+    hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, st, (const unsigned short *)x, (const unsigned short *)dy,
+                       dy_stride, d, D, chunk, dw);
+    int rc = launch_status();
+    if (rc == SRF_OK && db)
+        hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)cout), dim3(256), 0, st, (const unsigned short *)dy, V,
+                           dy_stride, db);
+    return rc == SRF_OK ? launch_status() : rc;
 }
 
 int srf_cnn_conv_wgrad(const void *x, const void *dy, int32_t dy_stride, int32_t D, int32_t H, int32_t W,
                        int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, float *dw, void *stream)
 {
-    if (!x || !dy || !dw || D <= 0 || H <= 0 || W <= 0) return SRF_ERR_INVALID_ARG;
-    if (cin_p <= 0 || cin_p % 8 || cout <= 0 || cout_p % 32 || cout > cout_p || dy_stride < cout)
-        return SRF_ERR_INVALID_ARG;
-    if (taps != 1 && taps != 27) return SRF_ERR_INVALID_ARG;
-    ConvDims d{D, H, W, cin_p, cout, cout_p, 0, 0, taps, -1.0f};
-    const int64_t V = (int64_t)D * H * W;
-    const int chunk = 4096;
-    dim3 grid((unsigned)((V + chunk - 1) / chunk), (unsigned)taps,
-              (unsigned)((cout_p / 32) * ((cin_p + 31) / 32)));
-    hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, (hipStream_t)stream, (const unsigned short *)x,
-                       (const unsigned short *)dy, dy_stride, d, chunk, dw);
-    return launch_status();
+    return srf_cnn_conv_wgrad_batch(x, dy, dy_stride, 1, D, H, W, cin_p, cout, cout_p, taps, dw, nullptr, stream);
 }
 
 int srf_cnn_flip_weights(const void *w, int32_t taps, int32_t cout, int32_t cout_p, int32_t cin_p, int32_t cin,

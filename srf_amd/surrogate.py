@@ -145,19 +145,21 @@ class UNet:
         return net, meta
 
     # --------------------------------------------------------------- buffers
-    def _buffers(self, D, H, W):
-        key = (D, H, W)
+    def _buffers(self, D, H, W, N=1):
+        """Activations of N samples of D x H x W, stacked along z (V[l] counts
+        the voxels of all N at level l; dims are one sample's)."""
+        key = (D, H, W, N)
         if key in self._bufs:
             return self._bufs[key]
         n = len(self.widths)
         if D % (1 << (n - 1)) or H % (1 << (n - 1)) or W % (1 << (n - 1)):
-            raise ValueError(f"grid {key} must be divisible by {1 << (n - 1)}")
+            raise ValueError(f"grid {key[:3]} must be divisible by {1 << (n - 1)}")
         dims = [(D >> l, H >> l, W >> l) for l in range(n)]
-        V = [d[0] * d[1] * d[2] for d in dims]
+        V = [N * d[0] * d[1] * d[2] for d in dims]
         bf = lambda *s: torch.zeros(s, dtype=torch.int16, device=self.dev)
         f32 = lambda *s: torch.zeros(s, dtype=torch.float32, device=self.dev)
         w = self.widths
-        B = {"dims": dims, "V": V}
+        B = {"dims": dims, "V": V, "N": N}
         for l in range(n):
             B[f"ea{l}"], B[f"e{l}"] = bf(V[l], w[l]), bf(V[l], w[l])
             B[f"gea{l}"], B[f"ge{l}"] = f32(V[l], w[l]), f32(V[l], w[l])
@@ -177,74 +179,82 @@ class UNet:
         return B
 
     # --------------------------------------------------------------- kernels
-    def _conv(self, L, x, dims, y_bf=None, y_f32=None, stride=None, off=0, act=True):
+    def _conv(self, L, x, dims, N, y_bf=None, y_f32=None, stride=None, off=0, act=True):
         D, H, W = dims
-        check(lib().srf_cnn_conv(x.data_ptr(), L.wbf.data_ptr(), L.b.data_ptr(), D, H, W, L.cin_p, L.cout, L.cout_p,
-                                 L.taps, stride or L.cout, off, self.slope if act else -1.0,
-                                 y_bf.data_ptr() if y_bf is not None else None,
-                                 y_f32.data_ptr() if y_f32 is not None else None, stream_handle(self.dev)),
-              "srf_cnn_conv " + L.name)
+        check(lib().srf_cnn_conv_batch(x.data_ptr(), L.wbf.data_ptr(), L.b.data_ptr(), N, D, H, W, L.cin_p, L.cout,
+                                       L.cout_p, L.taps, stride or L.cout, off, self.slope if act else -1.0,
+                                       y_bf.data_ptr() if y_bf is not None else None,
+                                       y_f32.data_ptr() if y_f32 is not None else None, stream_handle(self.dev)),
+              "srf_cnn_conv_batch " + L.name)
 
-    def _back(self, L, x, a, ga, ga_stride, dims, dp, gx=None, act=True):
-        """Gradients of layer L (input x, output a, dL/da = ga); gx <- dL/dx."""
+    def _back(self, L, x, a, ga, ga_stride, dims, N, dp, gx=None, act=True):
+        """Gradients of layer L (input x, output a, dL/da = ga); gx <- dL/dx.
+        The weight-gradient kernel also accumulates the bias gradient."""
         D, H, W = dims
-        V = D * H * W
+        V = N * D * H * W
         s = stream_handle(self.dev)
         check(lib().srf_cnn_act_bwd(ga.data_ptr(), ga_stride, a.data_ptr(), L.cout, V, L.cout, L.fcin_p,
                                     self.slope if act else -1.0, dp.data_ptr(), s), "srf_cnn_act_bwd")
-        check(lib().srf_cnn_conv_wgrad(x.data_ptr(), dp.data_ptr(), L.fcin_p, D, H, W, L.cin_p, L.cout, L.cout_p,
-                                       L.taps, L.gw.data_ptr(), s), "srf_cnn_conv_wgrad " + L.name)
-        check(lib().srf_cnn_bias_grad(dp.data_ptr(), V, L.cout, L.fcin_p, L.gb.data_ptr(), s), "srf_cnn_bias_grad")
+        check(lib().srf_cnn_conv_wgrad_batch(x.data_ptr(), dp.data_ptr(), L.fcin_p, N, D, H, W, L.cin_p, L.cout,
+                                             L.cout_p, L.taps, L.gw.data_ptr(), L.gb.data_ptr(), s),
+              "srf_cnn_conv_wgrad_batch " + L.name)
         if gx is not None:
-            check(lib().srf_cnn_conv(dp.data_ptr(), L.wf.data_ptr(), None, D, H, W, L.fcin_p, L.cin, L.fcout_p,
-                                     L.taps, L.cin, 0, -1.0, None, gx.data_ptr(), s), "srf_cnn_conv dgrad " + L.name)
+            check(lib().srf_cnn_conv_batch(dp.data_ptr(), L.wf.data_ptr(), None, N, D, H, W, L.fcin_p, L.cin,
+                                           L.fcout_p, L.taps, L.cin, 0, -1.0, None, gx.data_ptr(), s),
+                  "srf_cnn_conv_batch dgrad " + L.name)
 
     # ------------------------------------------------------------------ run
-    def forward(self, x0, dims):
-        """x0: int16 (V, 8) bf16 features (channels-last); dims (D, H, W).
-        Returns the float32 (V,) output buffer (reused by the next call)."""
-        B = self._buffers(*dims)
+    def forward(self, x0, dims, n_samples=1):
+        """x0: int16 (N V, 8) bf16 features (channels-last) of N = n_samples
+        samples stacked along z; dims (D, H, W) of one sample.  Returns the
+        float32 (N V,) output buffer (reused by the next call of that N)."""
+        N = int(n_samples)
+        B = self._buffers(*dims, N)
         n, w, s = len(self.widths), self.widths, stream_handle(self.dev)
         h = x0
         for l in range(n):
             d = B["dims"][l]
             if l > 0:
                 pd = B["dims"][l - 1]
-                check(lib().srf_cnn_pool(B[f"e{l - 1}"].data_ptr(), pd[0], pd[1], pd[2], w[l - 1], w[l - 1],
+                check(lib().srf_cnn_pool(B[f"e{l - 1}"].data_ptr(), N * pd[0], pd[1], pd[2], w[l - 1], w[l - 1],
                                          B[f"p{l - 1}"].data_ptr(), w[l - 1], s), "srf_cnn_pool")
                 h = B[f"p{l - 1}"]
-            self._conv(self.L[f"enc{l}a"], h, d, y_bf=B[f"ea{l}"])
-            self._conv(self.L[f"enc{l}b"], B[f"ea{l}"], d, y_bf=B[f"e{l}"])
+            self._conv(self.L[f"enc{l}a"], h, d, N, y_bf=B[f"ea{l}"])
+            self._conv(self.L[f"enc{l}b"], B[f"ea{l}"], d, N, y_bf=B[f"e{l}"])
         prev = B[f"e{n - 1}"]
         for l in range(n - 2, -1, -1):
             d, cd = B["dims"][l], B["dims"][l + 1]
             cu = w[l + 1] + w[l]
-            check(lib().srf_cnn_upsample(prev.data_ptr(), cd[0], cd[1], cd[2], w[l + 1], w[l + 1],
+            check(lib().srf_cnn_upsample(prev.data_ptr(), N * cd[0], cd[1], cd[2], w[l + 1], w[l + 1],
                                          B[f"u{l}"].data_ptr(), cu, 0, s), "srf_cnn_upsample")
             check(lib().srf_cnn_copy_channels(B[f"e{l}"].data_ptr(), B["V"][l], w[l], w[l], 0, B[f"u{l}"].data_ptr(),
                                               cu, w[l + 1], 0, 0, s), "srf_cnn_copy_channels")
-            self._conv(self.L[f"dec{l}a"], B[f"u{l}"], d, y_bf=B[f"da{l}"])
-            self._conv(self.L[f"dec{l}b"], B[f"da{l}"], d, y_bf=B[f"d{l}"])
+            self._conv(self.L[f"dec{l}a"], B[f"u{l}"], d, N, y_bf=B[f"da{l}"])
+            self._conv(self.L[f"dec{l}b"], B[f"da{l}"], d, N, y_bf=B[f"d{l}"])
             prev = B[f"d{l}"]
-        self._conv(self.L["head"], prev, B["dims"][0], y_f32=B["out"], stride=1, act=False)
+        self._conv(self.L["head"], prev, B["dims"][0], N, y_f32=B["out"], stride=1, act=False)
         self._x0 = x0
         return B["out"]
 
-    def backward(self, gout, dims):
-        """Accumulate parameter gradients for dL/d(output) = gout (float32 (V,))."""
-        B = self._buffers(*dims)
+    def backward(self, gout, dims, n_samples=1):
+        """Accumulate parameter gradients for dL/d(output) = gout (float32
+        (N V,)) of the last forward of N = n_samples samples."""
+        N = int(n_samples)
+        B = self._buffers(*dims, N)
         n, w, s = len(self.widths), self.widths, stream_handle(self.dev)
         dp = lambda l: B[f"dp{l}"]
         top = B["d0"] if n > 1 else B["e0"]
         gtop = B["gd0"] if n > 1 else B["ge0"]
-        self._back(self.L["head"], top, gout, gout, 1, B["dims"][0], dp(0), gx=gtop, act=False)
+        self._back(self.L["head"], top, gout, gout, 1, B["dims"][0], N, dp(0), gx=gtop, act=False)
         for l in range(0, n - 1):
             d, cd = B["dims"][l], B["dims"][l + 1]
             cu = w[l + 1] + w[l]
-            self._back(self.L[f"dec{l}b"], B[f"da{l}"], B[f"d{l}"], B[f"gd{l}"], w[l], d, dp(l), gx=B[f"gda{l}"])
-            self._back(self.L[f"dec{l}a"], B[f"u{l}"], B[f"da{l}"], B[f"gda{l}"], w[l], d, dp(l), gx=B[f"gu{l}"])
+            self._back(self.L[f"dec{l}b"], B[f"da{l}"], B[f"d{l}"], B[f"gd{l}"], w[l], d, N, dp(l),
+                       gx=B[f"gda{l}"])
+            self._back(self.L[f"dec{l}a"], B[f"u{l}"], B[f"da{l}"], B[f"gda{l}"], w[l], d, N, dp(l),
+                       gx=B[f"gu{l}"])
             dst = B[f"gd{l + 1}"] if l + 1 < n - 1 else B[f"ge{n - 1}"]
-            check(lib().srf_cnn_upsample_bwd(B[f"gu{l}"].data_ptr(), cd[0], cd[1], cd[2], w[l + 1], cu, 0,
+            check(lib().srf_cnn_upsample_bwd(B[f"gu{l}"].data_ptr(), N * cd[0], cd[1], cd[2], w[l + 1], cu, 0,
                                              dst.data_ptr(), w[l + 1], s), "srf_cnn_upsample_bwd")
         for l in range(n - 1, -1, -1):
             d = B["dims"][l]
@@ -253,19 +263,29 @@ class UNet:
                 check(lib().srf_cnn_copy_channels(B[f"gu{l}"].data_ptr(), B["V"][l], w[l], cu, w[l + 1],
                                                   B[f"ge{l}"].data_ptr(), w[l], 0, 1, 1, s), "srf_cnn_copy_channels")
             xin = self._x0 if l == 0 else B[f"p{l - 1}"]
-            self._back(self.L[f"enc{l}b"], B[f"ea{l}"], B[f"e{l}"], B[f"ge{l}"], w[l], d, dp(l), gx=B[f"gea{l}"])
-            self._back(self.L[f"enc{l}a"], xin, B[f"ea{l}"], B[f"gea{l}"], w[l], d, dp(l),
+            self._back(self.L[f"enc{l}b"], B[f"ea{l}"], B[f"e{l}"], B[f"ge{l}"], w[l], d, N, dp(l),
+                       gx=B[f"gea{l}"])
+            self._back(self.L[f"enc{l}a"], xin, B[f"ea{l}"], B[f"gea{l}"], w[l], d, N, dp(l),
                        gx=B[f"gp{l - 1}"] if l > 0 else None)
             if l > 0:
                 pd = B["dims"][l - 1]
-                check(lib().srf_cnn_pool_bwd(B[f"gp{l - 1}"].data_ptr(), pd[0], pd[1], pd[2], w[l - 1], w[l - 1],
-                                             B[f"ge{l - 1}"].data_ptr(), w[l - 1], s), "srf_cnn_pool_bwd")
+                check(lib().srf_cnn_pool_bwd(B[f"gp{l - 1}"].data_ptr(), N * pd[0], pd[1], pd[2], w[l - 1],
+                                             w[l - 1], B[f"ge{l - 1}"].data_ptr(), w[l - 1], s), "srf_cnn_pool_bwd")
 
-    def loss_and_grad(self, out, target, dims, weight=1.0):
-        """weight * MSE of out vs target into the loss buffer; returns dL/dout."""
-        B = self._buffers(*dims)
-        check(lib().srf_cnn_mse(out.data_ptr(), target.data_ptr(), out.numel(), float(weight), B["gout"].data_ptr(),
-                                B["loss"].data_ptr(), stream_handle(self.dev)), "srf_cnn_mse")
+    def loss_and_grad(self, out, target, dims, weight=1.0, n_samples=1):
+        """Sum over the N = n_samples samples of weight[i] * MSE(out_i,
+        target_i) into the loss buffer; returns dL/dout (N V,).  weight: one
+        number for every sample or a sequence of N."""
+        N = int(n_samples)
+        B = self._buffers(*dims, N)
+        ws = [float(weight)] * N if np.isscalar(weight) else [float(v) for v in weight]
+        if len(ws) != N:
+            raise ValueError(f"{len(ws)} weights for {N} samples")
+        V = B["V"][0] // N
+        s = stream_handle(self.dev)
+        for i, wi in enumerate(ws):
+            check(lib().srf_cnn_mse(out[i * V:].data_ptr(), target.reshape(-1)[i * V:].data_ptr(), V, wi,
+                                    B["gout"][i * V:].data_ptr(), B["loss"].data_ptr(), s), "srf_cnn_mse")
         return B["gout"]
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, gscale=1.0):
@@ -410,7 +430,8 @@ def train(net, X, T, dims, epochs, batch=4, lr=1e-3, seed=0, log=None, lr_min_fr
     N = X.shape[0]
     w = (1.0 / (T.double() ** 2).mean(1)).tolist() if relative else [1.0] * N
     g = torch.Generator().manual_seed(int(seed))
-    B = net._buffers(*dims)
+    nb = min(batch, N)
+    B = net._buffers(*dims, nb)
     steps_per_epoch = max(1, N // batch)
     total = epochs * steps_per_epoch
     hist, k, t0 = [], 0, time.time()
@@ -419,10 +440,11 @@ def train(net, X, T, dims, epochs, batch=4, lr=1e-3, seed=0, log=None, lr_min_fr
         B["loss"].zero_()
         for sidx in range(steps_per_epoch):
             net.zero_grad()
-            for i in perm[sidx * batch:(sidx + 1) * batch].tolist():
-                out = net.forward(X[i], dims)
-                gout = net.loss_and_grad(out, T[i], dims, w[i])
-                net.backward(gout, dims)
+            idx = perm[sidx * batch:(sidx + 1) * batch]
+            ix = idx.to(X.device)
+            out = net.forward(X.index_select(0, ix).reshape(-1, X.shape[-1]), dims, nb)
+            gout = net.loss_and_grad(out, T.index_select(0, ix), dims, [w[i] for i in idx.tolist()], nb)
+            net.backward(gout, dims, nb)
             frac = k / max(1, total - 1)
             cur = lr * (lr_min_frac + (1 - lr_min_frac) * 0.5 * (1 + math.cos(math.pi * frac)))
             net.adam(cur, gscale=1.0 / batch)
@@ -450,37 +472,40 @@ def flops_per_pass(net, dims):
 
 def _main():
     """Time forward and training passes of the surrogate network on random
-    bf16 features at the sinogram grid (32 x 64 x 64)."""
+    bf16 features at the sinogram grid (32 x 64 x 64), one sample and a batch
+    of samples per pass."""
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--dims", type=int, nargs=3, default=(32, 64, 64))
+    ap.add_argument("--batch", type=int, nargs="*", default=(1, 32))
     a = ap.parse_args()
     dev = torch.device("cuda", 0)
     dims = tuple(a.dims)
     V = dims[0] * dims[1] * dims[2]
     net = UNet(device=dev)
-    x = to_bf16(torch.randn((V, 8), device=dev))
-    t_ = torch.rand(V, device=dev)
-    for _ in range(2):
-        net.forward(x, dims)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        net.forward(x, dims)
-    torch.cuda.synchronize()
-    t_fwd = (time.perf_counter() - t0) / a.steps
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        net.zero_grad()
-        out = net.forward(x, dims)
-        net.backward(net.loss_and_grad(out, t_, dims), dims)
-        net.adam(1e-4)
-    torch.cuda.synchronize()
-    t_step = (time.perf_counter() - t0) / a.steps
     f = flops_per_pass(net, dims)
-    print(f"dims {dims}: forward {1e3 * t_fwd:.3f} ms ({f / t_fwd / 1e12:.1f} TFLOP/s dense), "
-          f"training step {1e3 * t_step:.3f} ms ({3 * f / t_step / 1e12:.1f} TFLOP/s)", flush=True)
+    for N in a.batch:
+        x = to_bf16(torch.randn((N * V, 8), device=dev))
+        t_ = torch.rand(N * V, device=dev)
+        for _ in range(2):
+            net.forward(x, dims, N)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            net.forward(x, dims, N)
+        torch.cuda.synchronize()
+        t_fwd = (time.perf_counter() - t0) / a.steps
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            net.zero_grad()
+            out = net.forward(x, dims, N)
+            net.backward(net.loss_and_grad(out, t_, dims, 1.0, N), dims, N)
+            net.adam(1e-4)
+        torch.cuda.synchronize()
+        t_step = (time.perf_counter() - t0) / a.steps
+        print(f"dims {dims} x {N}: forward {1e3 * t_fwd:.3f} ms ({N * f / t_fwd / 1e12:.1f} TFLOP/s dense), "
+              f"training step {1e3 * t_step:.3f} ms ({3 * N * f / t_step / 1e12:.1f} TFLOP/s)", flush=True)
 
 
 if __name__ == "__main__":

@@ -53,7 +53,9 @@ def _pad_w(w, cout_p, cin_p):
 
 
 @pytest.mark.parametrize("taps,dims,cin,cout", [(27, (4, 6, 10), 13, 40), (27, (2, 8, 8), 48, 16),
-                                                (1, (3, 5, 7), 16, 1), (27, (8, 4, 4), 128, 128)])
+                                                (1, (3, 5, 7), 16, 1), (27, (8, 4, 4), 128, 128),
+                                                (27, (4, 8, 16), 8, 16), (27, (2, 4, 40), 24, 96),
+                                                (27, (1, 1, 3), 8, 16)])
 def test_conv_forward(taps, dims, cin, cout):
     from srf_amd.surrogate import _c8, _c32
     rng = np.random.default_rng(taps + cin)
@@ -85,8 +87,12 @@ def test_conv_forward(taps, dims, cin, cout):
 
 
 @pytest.mark.parametrize("taps,dims,cin,cout", [(27, (4, 6, 10), 16, 40), (27, (8, 16, 16), 48, 16),
-                                                (1, (3, 5, 7), 16, 1)])
+                                                (1, (3, 5, 7), 16, 1), (27, (4, 8, 32), 32, 64),
+                                                (27, (4, 8, 8), 128, 128), (1, (4, 8, 16), 16, 1),
+                                                (27, (2, 4, 40), 24, 40)])
 def test_conv_weight_and_input_gradients(taps, dims, cin, cout):
+    """Shapes with W a multiple of 8 take the LDS-staged weight-gradient kernel
+    (W = 40: a partial second x tile), the others the element-load one."""
     from srf_amd.surrogate import _c8, _c32
     rng = np.random.default_rng(7 + taps)
     cin_p, cout_p = _c8(cin), _c32(cout)
@@ -119,6 +125,78 @@ def test_conv_weight_and_input_gradients(taps, dims, cin, cout):
     torch.cuda.synchronize()
     want = co.conv3d_grad_input(gr[..., :cout], wr[:, :cout, :cin]).reshape(V, cin)
     assert _rel(dx.double().cpu().numpy(), want) <= 1e-5
+
+
+@pytest.mark.parametrize("dims,cin,cout", [((4, 6, 16), 24, 40), ((4, 6, 10), 16, 16)])
+def test_batched_conv_and_gradients_per_sample(dims, cin, cout):
+    """Three samples stacked along z: the forward equals the oracle's
+    per-sample convolution (no tap reaches into the neighbouring sample) and
+    the weight and bias gradients equal the sums of the per-sample ones
+    (W = 16: the LDS kernel with the folded bias sum; W = 10: the element
+    path plus the bias kernel)."""
+    from srf_amd.surrogate import _c8, _c32
+    rng = np.random.default_rng(17)
+    N, taps = 3, 27
+    cin_p, cout_p, fcin_p = _c8(cin), _c32(cout), _c8(cout)
+    V = int(np.prod(dims))
+    x = np.zeros((N,) + dims + (cin_p,))
+    x[..., :cin] = rng.normal(size=(N,) + dims + (cin,))
+    xd, xr = _bf(x)
+    w = rng.normal(size=(taps, cout, cin)) / np.sqrt(taps * cin)
+    wd, wr = _bf(_pad_w(w, cout_p, cin_p))
+    b = rng.normal(size=cout).astype(np.float32)
+    y = torch.zeros((N * V, cout), dtype=torch.float32, device=DEV)
+    print(f"batched conv N={N} dims={dims} cin={cin} cout={cout}", flush=True)
+    assert _L().srf_cnn_conv_batch(xd.data_ptr(), wd.data_ptr(), torch.from_numpy(b).to(DEV).data_ptr(), N, *dims,
+                                   cin_p, cout, cout_p, taps, cout, 0, -1.0, None, y.data_ptr(), _s()) == 0
+    gy = np.zeros((N,) + dims + (fcin_p,))
+    gy[..., :cout] = rng.normal(size=(N,) + dims + (cout,))
+    gd, gr = _bf(gy)
+    dw = torch.zeros((taps, cout_p, cin_p), device=DEV)
+    db = torch.zeros(cout, device=DEV)
+    assert _L().srf_cnn_conv_wgrad_batch(xd.data_ptr(), gd.data_ptr(), fcin_p, N, *dims, cin_p, cout, cout_p, taps,
+                                         dw.data_ptr(), db.data_ptr(), _s()) == 0
+    torch.cuda.synchronize()
+    got = y.double().cpu().numpy().reshape(N, V, cout)
+    for i in range(N):
+        want = co.conv3d(xr[i], wr[:, :cout, :], b.astype(np.float64)).reshape(V, cout)
+        assert _rel(got[i], want) <= 1e-5, i
+    want_w = sum(co.conv3d_grad_weight(xr[i], gr[i][..., :cout], taps) for i in range(N))
+    assert _rel(dw.double().cpu().numpy()[:, :cout, :], want_w) <= 1e-5
+    assert np.all(dw.cpu().numpy()[:, cout:, :] == 0)
+    want_b = gr[..., :cout].reshape(-1, cout).sum(0)
+    assert _rel(db.double().cpu().numpy(), want_b) <= 1e-5
+
+
+def test_unet_batch_equals_per_sample():
+    """The U-Net over three stacked samples: outputs bit-identical to three
+    single-sample passes, parameter gradients equal to their sum (1e-5)."""
+    net, params, widths, xd, xr, t = _unet_case(12)
+    dims = (8, 8, 16)
+    V = int(np.prod(dims))
+    rng = np.random.default_rng(4)
+    N = 3
+    xs = np.zeros((N, V, 8))
+    xs[:, :, :4] = rng.normal(size=(N, V, 4))
+    X, _ = _bf(xs)
+    T = torch.from_numpy(rng.normal(size=(N, V)).astype(np.float32)).to(DEV)
+    print("unet batch N=3 dims", dims, flush=True)
+    singles, gsum = [], None
+    for i in range(N):
+        net.zero_grad()
+        out = net.forward(X[i], dims)
+        singles.append(out.clone())
+        net.backward(net.loss_and_grad(out, T[i], dims, 0.5 + i), dims)
+        g = net.grads()
+        gsum = g if gsum is None else {k: (gsum[k][0] + g[k][0], gsum[k][1] + g[k][1]) for k in g}
+    net.zero_grad()
+    outb = net.forward(X.reshape(N * V, 8), dims, N)
+    assert all(torch.equal(outb[i * V:(i + 1) * V], singles[i]) for i in range(N))
+    net.backward(net.loss_and_grad(outb, T, dims, [0.5 + i for i in range(N)], N), dims, N)
+    gb = net.grads()
+    torch.cuda.synchronize()
+    for k in gb:
+        assert _rel(gb[k][0], gsum[k][0]) <= 1e-5 and _rel(gb[k][1], gsum[k][1]) <= 1e-5, k
 
 
 def test_pool_upsample_copy_act_bias_mse_adam_cast():
