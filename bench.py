@@ -421,9 +421,9 @@ def batch_bench(args):
     eq = [float((a - b).norm() / b.norm()) for a, b in zip(imgs_b, imgs_p)]
     bitwise = all(bool(torch.equal(a, b)) for a, b in zip(imgs_b, imgs_p))
     vs_sss = [float((a - b).norm() / b.norm()) for a, b in zip(imgs_b, imgs_plain)]
-    checks = [e for _, e in est_b.checks]
-    log("surrogate checks (scan, rel L2 on the check ring): " +
-        ", ".join(f"{i}:{e:.3f}" for i, e in est_b.checks))
+    checks = [e for _, _, e in est_b.checks]
+    log("surrogate checks (scan/estimate: rel L2 on the check ring): " +
+        ", ".join(f"{i}/{k}:{e:.3f}" for i, k, e in est_b.checks))
     return {"workload": "32 scans, events log-uniform 1e5-2e7, in-plane {96,128,192,256}, 64-128 slices, "
                         "2-4 mm voxels, random ellipsoid phantoms, TOF FWHM 60 mm, OSEM 8 subsets x 4 "
                         "iterations, scatter before iterations 2-4",
@@ -432,7 +432,11 @@ def batch_bench(args):
             "speedup": t_plain / t_b, "per_scan_surrogate_s": t_p,
             "image_rel_l2_batched_vs_per_scan_max": max(eq), "image_bitwise_equal": bitwise,
             "image_rel_l2_surrogate_vs_sss_scatter_mean": float(np.mean(vs_sss)),
+            "image_rel_l2_surrogate_vs_sss_scatter_max": float(np.max(vs_sss)),
+            "checks": len(checks), "check_err_median": float(np.median(checks)) if checks else None,
+            "check_err_p95": float(np.percentile(checks, 95)) if checks else None,
             "check_err_max": max(checks) if checks else None,
+            "estimates_over_tolerance": sum(1 for e in checks if not e <= 0.02),
             "fallback_scans": sum(1 for v in est_b.fallback.values() if v),
             "autotuned": cfg.__dict__, "autotune_s": t_tune, "batch_info": info,
             "surrogate_weights": os.path.basename(wpath), "tof_window": TOF_WINDOW}

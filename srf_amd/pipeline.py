@@ -13,9 +13,10 @@ srfnef/corrections/scattering/scatter.py:51-53; srf_scatter_correct).
             (SingleScatterSimulation) for every scatter estimate
   batched   one BatchOsem over all scans (one launch per direction per subset,
             the iteration captured in a HIP graph, autotuned schedule); the CNN
-            surrogate for the scatter estimates, checked against SSS on a few
-            rings at each scan's first estimate -- a scan whose check error
-            exceeds the tolerance uses SSS from then on (the fallback)
+            surrogate for the scatter estimates, every estimate checked against
+            SSS on a few rings -- an estimate whose check error exceeds the
+            tolerance is replaced by the full SSS, and that scan uses SSS from
+            then on (the fallback)
 
 Scans follow the batched-reconstruction workload: event counts log-uniform in
 [1e5, 2e7], in-plane sizes {96, 128, 192, 256}, 64-128 slices, isotropic
@@ -128,7 +129,8 @@ class ScatterEstimator:
         self.check_rings, self.tol, self.fraction = tuple(check_rings), float(tol), float(fraction)
         self._sss, self._chk, self._feat = {}, {}, {}
         self.fallback = {}        # scan key -> True once a check failed
-        self.checks = []          # (scan key, error)
+        self.checks = []          # (scan key, estimate index of that scan, error)
+        self._n_est = {}          # scan key -> estimates made
 
     @staticmethod
     def _key(sc):
@@ -144,29 +146,68 @@ class ScatterEstimator:
             self._sss[k] = SingleScatterSimulation(self.g, sc.shape, sc.center, sc.size, cell_normalized=True)
         return self._sss[k](x_xyz, sc.meta["mu"])
 
-    def surrogate(self, sc, x_xyz, check, scan_id):
-        if self.fallback.get(scan_id):
-            return self._sss_of(sc, body_estimate(x_xyz, sc.meta["mu"], sc.meta.get("voxel"))), True
-        x_xyz = body_estimate(x_xyz, sc.meta["mu"], sc.meta.get("voxel"))
+    def surrogate(self, sc, x_xyz, scan_id):
+        """One scan's surrogate estimate, checked (surrogate_batch of one)."""
+        return self.surrogate_batch([(scan_id, sc, x_xyz)])[0]
+
+    def _features(self, sc):
         k = self._key(sc)
         if k not in self._feat:
             self._feat[k] = SinogramFeatures(self.g, sc.shape, sc.center, sc.size, self.net.dev,
                                              coarse_spacing=self.coarse_spacing)
-        feat = self._feat[k]
-        xf, m = feat(x_xyz, sc.meta["mu"])
-        out = self.net.forward(xf, feat.dims)
-        pred = (out * (m / self.kappa)).reshape(feat.dims)
-        if check and self.check_rings:
-            if k not in self._chk:
-                self._chk[k] = SingleScatterSimulation(self.g, sc.shape, sc.center, sc.size, rings=self.check_rings,
-                                                       cell_normalized=True)
-            ref = self._chk[k](x_xyz, sc.meta["mu"])
-            err = relative_l2(pred[list(self.check_rings)], ref)
-            self.checks.append((scan_id, err))
-            if not err <= self.tol:
-                self.fallback[scan_id] = True
-                return self._sss_of(sc, x_xyz), True
-        return pred, False
+        return self._feat[k]
+
+    def _checker(self, sc):
+        k = self._key(sc)
+        if k not in self._chk:
+            self._chk[k] = SingleScatterSimulation(self.g, sc.shape, sc.center, sc.size, rings=self.check_rings,
+                                                   cell_normalized=True)
+        return self._chk[k]
+
+    def surrogate_batch(self, items):
+        """Surrogate estimates of several scans at once: per scan the body
+        estimate, the features (written into one stacked input) and the
+        check-ring SSS; then one CNN forward over all of them, every check
+        error on the GPU and a single transfer of the errors to the host.
+        items: [(scan_id, scan, x_xyz)].  Returns [(sinogram, fell_back)] in
+        item order; scans whose check fails (now or before) get the full SSS."""
+        res = [None] * len(items)
+        todo = []
+        for j, (scan_id, sc, x_xyz) in enumerate(items):
+            xb = body_estimate(x_xyz, sc.meta["mu"], sc.meta.get("voxel"))
+            if self.fallback.get(scan_id):
+                res[j] = (self._sss_of(sc, xb), True)
+            else:
+                todo.append((j, scan_id, sc, xb))
+        if not todo:
+            return res
+        dims = (self.g.n_rings, self.g.n_angles, self.g.n_radial)
+        n = dims[0] * dims[1] * dims[2]
+        X = torch.empty((len(todo) * n, 8), dtype=torch.int16, device=self.net.dev)
+        ms, refs = [], []
+        for i, (j, scan_id, sc, xb) in enumerate(todo):
+            _, m = self._features(sc)(xb, sc.meta["mu"], out=X[i * n:(i + 1) * n])
+            ms.append(m)
+            if self.check_rings:
+                refs.append(self._checker(sc)(xb, sc.meta["mu"]))
+        out = self.net.forward(X, dims, len(todo))
+        preds = out.reshape(len(todo), *dims) * (torch.cat(ms) / self.kappa).reshape(-1, 1, 1, 1)
+        if self.check_rings:
+            rings = list(self.check_rings)
+            errs = torch.stack([torch.linalg.vector_norm((preds[i, rings] - r).double())
+                                / torch.linalg.vector_norm(r.double()).clamp_min(1e-300)
+                                for i, r in enumerate(refs)]).cpu().tolist()
+        for i, (j, scan_id, sc, xb) in enumerate(todo):
+            fell = False
+            if self.check_rings:
+                nn_ = self._n_est.get(scan_id, 0)
+                self._n_est[scan_id] = nn_ + 1
+                self.checks.append((scan_id, nn_, errs[i]))
+                if not errs[i] <= self.tol:
+                    self.fallback[scan_id] = True
+                    fell = True
+            res[j] = (self._sss_of(sc, xb), True) if fell else (preds[i], False)
+        return res
 
     def values(self, sc, sino):
         return scatter_correct(None, sc.meta["inv_att"], sc.meta["bins"], sino, self.fraction)
@@ -197,7 +238,7 @@ def reconstruct_plain(scans, n_sub=8, n_iter=4, estimator=None, scatter="sss", i
                 if scatter == "sss":
                     sino = estimator.sss(sc, x)
                 else:
-                    sino, _ = estimator.surrogate(sc, x, it == 1, i)
+                    sino, _ = estimator.surrogate(sc, x, i)
                 eng.y.copy_(estimator.values(sc, sino).index_select(0, pidx))
                 torch.cuda.synchronize()
                 t_sc += time.time() - ts
@@ -233,13 +274,13 @@ def reconstruct_batched(scans, n_sub=8, n_iter=4, estimator=None, config=BatchCo
         if it >= 1 and scatter is not None:
             torch.cuda.synchronize()
             ts = time.time()
-            for i, sc in enumerate(scans):
-                x = b.image(i)
-                if scatter == "sss":
-                    sino = estimator.sss(sc, x)
-                else:
-                    sino, _ = estimator.surrogate(sc, x, it == 1, i)
-                b.set_values(i, estimator.values(sc, sino))
+            if scatter == "sss":
+                for i, sc in enumerate(scans):
+                    b.set_values(i, estimator.values(sc, estimator.sss(sc, b.image(i))))
+            else:
+                est = estimator.surrogate_batch([(i, sc, b.image(i)) for i, sc in enumerate(scans)])
+                for i, sc in enumerate(scans):
+                    b.set_values(i, estimator.values(sc, est[i][0]))
             torch.cuda.synchronize()
             t_sc += time.time() - ts
         b.advance(graph)

@@ -261,7 +261,10 @@ class SingleScatterSimulation:
         need = int(lib().srf_sss_rows_bytes(self.nd, max(ns, 1))) // 4 + 4
         return torch.empty(need, dtype=torch.float32, device=self.dev)
 
-    MU_CACHE_MAX = 1 << 26   # largest kept set of attenuation integrals (floats)
+    # largest kept set of attenuation integrals (floats): 2^28 (1 GiB) holds a
+    # full set of 4096 detectors x 6.5e4 scatter points, so repeated estimates
+    # for one mu-map (every OSEM iteration of a scan) skip half the projections
+    MU_CACHE_MAX = 1 << 28
 
     def _mu_part(self, mu_xyz):
         """The scatter points and the attenuation line integrals of mu_xyz.

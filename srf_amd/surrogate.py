@@ -162,21 +162,38 @@ class UNet:
         B = {"dims": dims, "V": V, "N": N}
         for l in range(n):
             B[f"ea{l}"], B[f"e{l}"] = bf(V[l], w[l]), bf(V[l], w[l])
+            if l + 1 < n:
+                B[f"p{l}"] = bf(V[l + 1], w[l])
+                B[f"u{l}"] = bf(V[l], w[l + 1] + w[l])
+                B[f"da{l}"], B[f"d{l}"] = bf(V[l], w[l]), bf(V[l], w[l])
+        B["out"] = f32(V[0])
+        self._bufs[key] = B
+        return B
+
+    def _grad_buffers(self, D, H, W, N=1):
+        """_buffers plus the gradient buffers (allocated on the first backward
+        or loss of that size, so that inference holds activations only)."""
+        B = self._buffers(D, H, W, N)
+        if "gout" in B:
+            return B
+        n, w, V = len(self.widths), self.widths, B["V"]
+        bf = lambda *s: torch.zeros(s, dtype=torch.int16, device=self.dev)
+        f32 = lambda *s: torch.zeros(s, dtype=torch.float32, device=self.dev)
+        for l in range(n):
             B[f"gea{l}"], B[f"ge{l}"] = f32(V[l], w[l]), f32(V[l], w[l])
             cmax = max(_c8(c) for c in [w[l], 1] + ([w[l + 1]] if l + 1 < n else []))
             B[f"dp{l}"] = bf(V[l], max(cmax, 8))
             if l + 1 < n:
-                B[f"p{l}"] = bf(V[l + 1], w[l])
                 B[f"gp{l}"] = f32(V[l + 1], w[l])
-                cu = w[l + 1] + w[l]
-                B[f"u{l}"], B[f"gu{l}"] = bf(V[l], cu), f32(V[l], cu)
-                B[f"da{l}"], B[f"d{l}"] = bf(V[l], w[l]), bf(V[l], w[l])
+                B[f"gu{l}"] = f32(V[l], w[l + 1] + w[l])
                 B[f"gda{l}"], B[f"gd{l}"] = f32(V[l], w[l]), f32(V[l], w[l])
-        B["out"] = f32(V[0])
         B["gout"] = f32(V[0])
         B["loss"] = f32(1)
-        self._bufs[key] = B
         return B
+
+    def release(self):
+        """Drop every cached activation and gradient buffer."""
+        self._bufs.clear()
 
     # --------------------------------------------------------------- kernels
     def _conv(self, L, x, dims, N, y_bf=None, y_f32=None, stride=None, off=0, act=True):
@@ -240,7 +257,7 @@ class UNet:
         """Accumulate parameter gradients for dL/d(output) = gout (float32
         (N V,)) of the last forward of N = n_samples samples."""
         N = int(n_samples)
-        B = self._buffers(*dims, N)
+        B = self._grad_buffers(*dims, N)
         n, w, s = len(self.widths), self.widths, stream_handle(self.dev)
         dp = lambda l: B[f"dp{l}"]
         top = B["d0"] if n > 1 else B["e0"]
@@ -277,7 +294,7 @@ class UNet:
         target_i) into the loss buffer; returns dL/dout (N V,).  weight: one
         number for every sample or a sequence of N."""
         N = int(n_samples)
-        B = self._buffers(*dims, N)
+        B = self._grad_buffers(*dims, N)
         ws = [float(weight)] * N if np.isscalar(weight) else [float(v) for v in weight]
         if len(ws) != N:
             raise ValueError(f"{len(ws)} weights for {N} samples")
@@ -357,8 +374,10 @@ class SinogramFeatures:
         self._mu_cache = (mu_xyz, mu_xyz._version, mu)
         return mu
 
-    def __call__(self, em_xyz, mu_xyz):
-        """-> (x (n, 8) int16 bf16, m (1,) float32 mean emission integral)."""
+    def __call__(self, em_xyz, mu_xyz, out=None):
+        """-> (x (n, 8) int16 bf16, m (1,) float32 mean emission integral);
+        out: a contiguous (n, 8) int16 tensor to write x into (a slice of a
+        batch's input)."""
         em = self.integrals(em_xyz)
         mu = self._mu_integrals(mu_xyz)
         m = em.mean().reshape(1).clamp_min(1e-30)
@@ -367,7 +386,12 @@ class SinogramFeatures:
         if self.coarse is not None:
             extra = self.coarse(em_xyz.to(self.dev, torch.float32), mu_xyz.to(self.dev, torch.float32))
             extra = extra.reshape(-1).contiguous()
-        x = torch.empty((self.n, 8), dtype=torch.int16, device=self.dev)
+        if out is not None:
+            if out.shape != (self.n, 8) or out.dtype != torch.int16 or not out.is_contiguous():
+                raise ValueError("out must be a contiguous (n, 8) int16 tensor")
+            x = out
+        else:
+            x = torch.empty((self.n, 8), dtype=torch.int16, device=self.dev)
         check(lib().srf_cnn_features(em.data_ptr(), mu.data_ptr(), extra.data_ptr() if extra is not None else None,
                                      self.n, scale.data_ptr(), x.data_ptr(), stream_handle(self.dev)),
               "srf_cnn_features")
@@ -431,7 +455,7 @@ def train(net, X, T, dims, epochs, batch=4, lr=1e-3, seed=0, log=None, lr_min_fr
     w = (1.0 / (T.double() ** 2).mean(1)).tolist() if relative else [1.0] * N
     g = torch.Generator().manual_seed(int(seed))
     nb = min(batch, N)
-    B = net._buffers(*dims, nb)
+    B = net._grad_buffers(*dims, nb)
     steps_per_epoch = max(1, N // batch)
     total = epochs * steps_per_epoch
     hist, k, t0 = [], 0, time.time()

@@ -30,7 +30,9 @@ def test_untrained_surrogate_falls_back_to_sss_bit_for_bit():
     imgs_b, t_b, info = reconstruct_batched(scans, n_sub=2, n_iter=3, estimator=est,
                                             config=BatchConfig(persistent=1), graph=True)
     print("checks", est.checks, "fallback", est.fallback, info, flush=True)
-    assert len(est.checks) == len(scans) and all(e > 0.02 for _, e in est.checks)
+    # each scan's first estimate is checked, fails and switches that scan to SSS
+    assert len(est.checks) == len(scans) and all(e > 0.02 for _, _, e in est.checks)
+    assert sorted(i for i, _, _ in est.checks) == list(range(len(scans)))
     assert all(est.fallback.get(i) for i in range(len(scans)))
     imgs_p, t_p = reconstruct_plain(scans, n_sub=2, n_iter=3, estimator=_estimator(0.02), scatter="sss")
     for a, b in zip(imgs_b, imgs_p):
@@ -44,6 +46,8 @@ def test_batched_surrogate_loop_matches_per_scan_loop():
     est_b = _estimator(math.inf)
     imgs_b, _, _ = reconstruct_batched(scans, n_sub=2, n_iter=3, estimator=est_b, config=BatchConfig(), graph=True)
     assert not any(est_b.fallback.values())
+    # every in-loop estimate (iterations 2 and 3 of every scan) is checked
+    assert sorted((i, k) for i, k, _ in est_b.checks) == [(i, k) for i in range(len(scans)) for k in range(2)]
     est_p = _estimator(math.inf)
     imgs_p, _ = reconstruct_plain(scans, n_sub=2, n_iter=3, estimator=est_p, scatter="surrogate")
     for a, b in zip(imgs_b, imgs_p):
