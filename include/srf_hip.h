@@ -356,6 +356,14 @@ int srf_cnn_conv(const void *x, const void *w, const float *bias, int32_t D, int
 int srf_cnn_conv_batch(const void *x, const void *w, const float *bias, int32_t N, int32_t D, int32_t H, int32_t W,
                        int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, int32_t out_stride, int32_t out_off,
                        float slope, void *y_bf16, float *y_f32, void *stream);
+/* srf_cnn_conv_batch of the concatenation [upsample2x(coarse), fine] read in
+ * place: input channels [0, cc) are coarse (rows of cc channels on the
+ * (D/2, H/2, W/2) grid of each sample, nearest neighbour), [cc, cin_p) are
+ * fine (rows of cin_p - cc channels).  D, H, W even; cc a multiple of 8. */
+int srf_cnn_conv_up_batch(const void *fine, const void *coarse, int32_t cc, const void *w, const float *bias,
+                          int32_t N, int32_t D, int32_t H, int32_t W, int32_t cin_p, int32_t cout, int32_t cout_p,
+                          int32_t taps, int32_t out_stride, int32_t out_off, float slope, void *y_bf16, float *y_f32,
+                          void *stream);
 /* dw[t][n][c] += sum_v dy[v][n] x[v + o_t][c]   (dy bf16 rows of dy_stride);
  * the _batch form over N stacked samples, and with db != NULL also
  * db[n] += sum_v dy[v][n] (the bias gradient, as srf_cnn_bias_grad). */

@@ -35,14 +35,8 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-__device__ __forceinline__ unsigned short f2bf(float f)
-{
-    // round to nearest even (NaN kept as a quiet NaN)
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
+// float -> bf16 bits, round to nearest even (gfx950's v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 
 struct ConvDims {
     int D, H, W;      // spatial size (same in and out)
@@ -81,10 +75,34 @@ __device__ __forceinline__ void tap_offset(int t, int taps, int &dz, int &dy, in
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct ConvTile {
-    int tx, ty, tz;    // tile extent
-    int ntx, nty;      // tiles along x and y
-    int Ds;            // slices per sample (tz divides it)
+    int tx, ty, tz;             // tile extent
+    int ntx, nty;               // tiles along x and y
+    int Ds;                     // slices per sample (tz divides it)
+    int nblk;                   // tiles of the launch
+    float inv_hx, inv_hyx;      // 1 / halo row, 1 / halo plane (voxels)
+    float inv_tx, inv_txy;      // 1 / tile row, 1 / tile plane
 };
+
+// The input of a convolution: channels [0, cc) are the nearest-neighbour 2x
+// upsampling of `coarse` (rows of cc channels on the half-size grid) and
+// channels [cc, cin_p) come from `fine` (rows of cin_p - cc channels): the
+// decoder's concatenation read in place.  cc = 0: every channel from fine.
+struct ConvIn {
+    const unsigned short *fine;
+    const unsigned short *coarse;
+    int cc;
+};
+
+// floor(a / b) for 0 <= a < 2^24 and b >= 1 given inv = 1/b in float: the
+// float quotient is off by at most one, and one integer step corrects it.
+__device__ __forceinline__ int idiv(int a, int b, float inv)
+{
+    int q = (int)((float)a * inv);
+    const int r = a - q * b;
+    q += r >= b ? 1 : 0;
+    q -= r < 0 ? 1 : 0;
+    return q;
+}
 constexpr int CONV_TILE = 128;
 constexpr int CONV_LDS_MAX = 65536;
 
@@ -102,30 +120,24 @@ __device__ __forceinline__ void conv_chunk(const u16x8 *__restrict__ halo, const
 {
     constexpr int G = TAPS * NG;          // K groups of 8 in this chunk
     constexpr int S = (G + 3) / 4;        // MFMA steps (4 groups each)
+    constexpr int K = 4 / NG;             // taps per step
+    // lane l's K group q is (tap K s + q / NG, channel group q % NG)
+    const int g = q & (NG - 1), tq = q / NG;
+    const u16x8 *a0 = halo + hb[0] * gp + g, *a1 = halo + hb[1] * gp + g;
+    const unsigned short *wq = wl + tq * tap_stride + 8 * g;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        int t, g;
-        if constexpr (NG == 4) {
-            t = s;
-            g = q;
-        } else if constexpr (NG == 2) {
-            t = 2 * s + (q >> 1);
-            g = q & 1;
-        } else {
-            t = 4 * s + q;
-            g = 0;
-        }
-        // tap offsets: uniform candidates, one picked per lane
+        // the tap offsets of this step: uniform candidates, one picked per lane
         int delta;
-        if constexpr (NG == 4) {
+        if constexpr (K == 1) {
             delta = tap_delta(s, TAPS, hx, hyx);
-        } else if constexpr (NG == 2) {
+        } else if constexpr (K == 2) {
             const int d0 = tap_delta(2 * s, TAPS, hx, hyx), d1 = tap_delta(2 * s + 1, TAPS, hx, hyx);
-            delta = (q >> 1) ? d1 : d0;
+            delta = tq ? d1 : d0;
         } else {
             const int d0 = tap_delta(4 * s, TAPS, hx, hyx), d1 = tap_delta(4 * s + 1, TAPS, hx, hyx);
             const int d2 = tap_delta(4 * s + 2, TAPS, hx, hyx), d3 = tap_delta(4 * s + 3, TAPS, hx, hyx);
-            delta = q == 0 ? d0 : q == 1 ? d1 : q == 2 ? d2 : d3;
+            delta = tq == 0 ? d0 : tq == 1 ? d1 : tq == 2 ? d2 : d3;
         }
         const bool valid = (G % 4 == 0 || s < S - 1) || (4 * s + q < G);
         u16x8 a[2], w[NBW];
@@ -134,11 +146,11 @@ __device__ __forceinline__ void conv_chunk(const u16x8 *__restrict__ halo, const
 #pragma unroll
         for (int j = 0; j < NBW; ++j) w[j] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
         if (valid) {
+            a[0] = a0[delta * gp];
+            a[1] = a1[delta * gp];
+            const unsigned short *ws = wq + (int64_t)(K * s) * tap_stride;
 #pragma unroll
-            for (int r = 0; r < 2; ++r) a[r] = halo[(hb[r] + delta) * gp + g];
-#pragma unroll
-            for (int j = 0; j < NBW; ++j)
-                w[j] = *reinterpret_cast<const u16x8 *>(wl + t * tap_stride + j * nb_stride + 8 * g);
+            for (int j = 0; j < NBW; ++j) w[j] = *reinterpret_cast<const u16x8 *>(ws + j * nb_stride);
         }
 #pragma unroll
         for (int r = 0; r < 2; ++r)
@@ -151,32 +163,35 @@ __device__ __forceinline__ void conv_chunk(const u16x8 *__restrict__ halo, const
 }
 
 template <int TAPS, int NBW>
-__global__ __launch_bounds__(256) void k_conv(const unsigned short *__restrict__ X,
-                                              const unsigned short *__restrict__ Wt,
+__global__ __launch_bounds__(256) void k_conv(ConvIn in, const unsigned short *__restrict__ Wt,
                                               const float *__restrict__ bias, ConvDims d, ConvTile T,
                                               unsigned short *__restrict__ Ybf, float *__restrict__ Yf)
 {
     constexpr int R = TAPS == 27 ? 1 : 0;
+    constexpr int STAGE = 4;   // halo loads in flight per thread
     extern __shared__ u16x8 halo[];
+    // the hardware deals workgroup i to XCD i % 8: give every XCD a contiguous
+    // run of tiles, so that the halos neighbouring tiles share hit its L2
+    const int per = gridDim.x >> 3;
+    const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (b >= T.nblk) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, q = lane >> 4;
-    int b = blockIdx.x;
-    const int bx = b % T.ntx;
-    b /= T.ntx;
-    const int by = b % T.nty;
-    const int bz = b / T.nty;
+    const int bx = b % T.ntx, by = (b / T.ntx) % T.nty, bz = b / (T.ntx * T.nty);
     const int x0 = bx * T.tx, y0 = by * T.ty, z0 = bz * T.tz;
     const int zs0 = (z0 / T.Ds) * T.Ds;   // first slice of the tile's sample
     const int hx = T.tx + 2 * R, hy = T.ty + 2 * R, hz = T.tz + 2 * R;
     const int hyx = hx * hy, HV = hyx * hz;
-    const int nvox = T.tx * T.ty * T.tz;
+    const int txy = T.tx * T.ty, nvox = txy * T.tz;
     const int n0 = blockIdx.y * 16 * NBW;
+    const int Hc = d.H >> 1, Wc = d.W >> 1;
     // this lane's A rows (clamped into the tile; rows past it are not stored)
     int hb[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         const int tv = min(wave * 32 + r * 16 + col, nvox - 1);
-        const int ltx = tv % T.tx, lty = (tv / T.tx) % T.ty, ltz = tv / (T.tx * T.ty);
+        const int ltz = idiv(tv, txy, T.inv_txy), rem = tv - ltz * txy;
+        const int lty = idiv(rem, T.tx, T.inv_tx), ltx = rem - lty * T.tx;
         hb[r] = (ltz * hy + lty) * hx + ltx;
     }
     f32x4 acc[2][NBW];
@@ -187,18 +202,51 @@ __global__ __launch_bounds__(256) void k_conv(const unsigned short *__restrict__
     const int64_t tap_stride = (int64_t)d.cout_p * d.cin_p;
     const int nb_stride = 16 * d.cin_p;
     for (int c0 = 0; c0 < d.cin_p;) {
-        const int left = (d.cin_p - c0) >> 3;
+        const bool up = c0 < in.cc;   // a chunk lies wholly in one of the two inputs
+        const int left = ((up ? in.cc : d.cin_p) - c0) >> 3;
         const int lg = left >= 4 ? 2 : left >= 2 ? 1 : 0;   // log2 of the chunk's 8-channel groups
         const int ng = 1 << lg, gp = ng | 1;
+        const unsigned short *src = up ? in.coarse + c0 : in.fine + (c0 - in.cc);
+        const int stride = up ? in.cc : d.cin_p - in.cc;
+        // thread i stages items i, i + 256, ... (item = 8-channel group g of
+        // halo voxel hv); the halo coordinates advance by a fixed step per pass
+        const int g = threadIdx.x & (ng - 1);
+        const int dv = 256 >> lg;                                  // voxels per pass
+        const int sa = dv / hyx, sb = (dv - sa * hyx) / hx, sc = dv - sa * hyx - sb * hx;
+        int hv = threadIdx.x >> lg;
+        int hzi = idiv(hv, hyx, T.inv_hyx);
+        int hyi = idiv(hv - hzi * hyx, hx, T.inv_hx);
+        int hxi = hv - hzi * hyx - hyi * hx;
         __syncthreads();   // the previous chunk's reads are done
-        for (int i = threadIdx.x; i < (HV << lg); i += 256) {
-            const int g = i & (ng - 1), hv = i >> lg;
-            const int gx = x0 + hv % hx - R, gy = y0 + (hv / hx) % hy - R, gz = z0 + hv / hyx - R;
-            u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= zs0 && gz < zs0 + T.Ds)
-                val = *reinterpret_cast<const u16x8 *>(X + ((int64_t)(gz * d.H + gy) * d.W + gx) * d.cin_p + c0 +
-                                                       8 * g);
-            halo[hv * gp + g] = val;
+        while (hv < HV) {
+            u16x8 v[STAGE];
+            int dst[STAGE];
+#pragma unroll
+            for (int u = 0; u < STAGE; ++u) {
+                v[u] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                dst[u] = hv < HV ? hv * gp + g : -1;
+                const int gx = x0 + hxi - R, gy = y0 + hyi - R, gz = z0 + hzi - R;
+                if (hv < HV && (unsigned)gx < (unsigned)d.W && (unsigned)gy < (unsigned)d.H && gz >= zs0 &&
+                    gz < zs0 + T.Ds) {
+                    const int vox = up ? ((gz >> 1) * Hc + (gy >> 1)) * Wc + (gx >> 1) : (gz * d.H + gy) * d.W + gx;
+                    v[u] = *reinterpret_cast<const u16x8 *>(src + ((unsigned)vox * (unsigned)stride + 8 * g));
+                }
+                hv += dv;
+                hxi += sc;
+                hyi += sb;
+                hzi += sa;
+                if (hxi >= hx) {
+                    hxi -= hx;
+                    ++hyi;
+                }
+                if (hyi >= hy) {
+                    hyi -= hy;
+                    ++hzi;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < STAGE; ++u)
+                if (dst[u] >= 0) halo[dst[u]] = v[u];
         }
         __syncthreads();
         const unsigned short *wl = Wt + (int64_t)(n0 + col) * d.cin_p + c0;
@@ -210,26 +258,79 @@ __global__ __launch_bounds__(256) void k_conv(const unsigned short *__restrict__
             conv_chunk<TAPS, 1, NBW>(halo, hb, gp, hx, hyx, wl, tap_stride, nb_stride, q, acc);
         c0 += 8 * ng;
     }
-    // C layout of a 16 x 16 block: lane l holds column l % 16, rows 4 * (l / 16) + i
+    // Epilogue through LDS: the wavefront's 32 x 16 NBW block (bias and
+    // activation applied) goes to LDS in the MFMA layout (lane l holds column
+    // l % 16, rows 4 (l / 16) + i of each 16 x 16 block) and comes back row by
+    // row, so that consecutive lanes store consecutive bytes: 16-byte groups
+    // of 8 bf16 / 4 float channels where the row layout allows, else single
+    // values (the 1-channel head: one value per voxel, consecutive voxels).
+    constexpr int RL = 16 * NBW + 1;   // floats per LDS row
+    __syncthreads();                   // every wavefront is done with the halo
+    float *cbuf = reinterpret_cast<float *>(halo) + wave * (32 * RL + 64);
+    int64_t *orow = reinterpret_cast<int64_t *>(cbuf + 32 * RL);   // row -> global offset
 #pragma unroll
     for (int j = 0; j < NBW; ++j) {
         const int n = n0 + 16 * j + col;
-        if (n >= d.cout) continue;
-        const float bn = bias ? bias[n] : 0.0f;
+        const float bn = (bias && n < d.cout) ? bias[n] : 0.0f;
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int tv = wave * 32 + r * 16 + 4 * q + i;
-                if (tv >= nvox) continue;
-                const int gx = x0 + tv % T.tx, gy = y0 + (tv / T.tx) % T.ty, gz = z0 + tv / (T.tx * T.ty);
-                if (gx >= d.W || gy >= d.H) continue;
                 float v = acc[r][j][i] + bn;
                 if (d.slope >= 0.0f && v < 0.0f) v *= d.slope;
-                const int64_t o = ((int64_t)(gz * d.H + gy) * d.W + gx) * d.out_stride + d.out_off + n;
-                if (Ybf) Ybf[o] = f2bf(v);
-                if (Yf) Yf[o] = v;
+                cbuf[(r * 16 + 4 * q + i) * RL + 16 * j + col] = v;
             }
+    }
+    // global offset of each row of this wavefront, -1 outside the volume
+    if (lane < 32) {
+        const int tv = wave * 32 + lane;
+        const int ltz = idiv(tv, txy, T.inv_txy), rem = tv - ltz * txy;
+        const int lty = idiv(rem, T.tx, T.inv_tx), ltx = rem - lty * T.tx;
+        const int gx = x0 + ltx, gy = y0 + lty, gz = z0 + ltz;
+        orow[lane] = (tv < nvox && gx < d.W && gy < d.H)
+                         ? ((int64_t)(gz * d.H + gy) * d.W + gx) * d.out_stride + d.out_off + n0
+                         : -1;
+    }
+    __syncthreads();   // LDS writes visible to the whole wavefront
+    const int cvalid = min(16 * NBW, d.cout - n0);   // channels of this block to store
+    const bool vec_bf = Ybf && (cvalid % 8 == 0) && ((d.out_stride | d.out_off | n0) % 8 == 0) &&
+                        !(reinterpret_cast<uintptr_t>(Ybf) & 15);
+    const bool vec_f = Yf && (cvalid % 4 == 0) && ((d.out_stride | d.out_off | n0) % 4 == 0) &&
+                       !(reinterpret_cast<uintptr_t>(Yf) & 15);
+    if (Ybf) {
+        const int per = vec_bf ? cvalid >> 3 : cvalid;          // items per row
+        const float inv = 1.0f / (float)per;
+        const int sh = (per & (per - 1)) ? -1 : __builtin_ctz(per);
+        for (int e = lane; e < 32 * per; e += 64) {
+            const int row = sh >= 0 ? e >> sh : idiv(e, per, inv), k = e - row * per;
+            const int64_t o = orow[row];
+            if (o < 0) continue;
+            const float *src = cbuf + row * RL;
+            if (vec_bf) {
+                u16x8 w;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) w[c] = f2bf(src[8 * k + c]);
+                *reinterpret_cast<u16x8 *>(Ybf + o + 8 * k) = w;
+            } else {
+                Ybf[o + k] = f2bf(src[k]);
+            }
+        }
+    }
+    if (Yf) {
+        const int per = vec_f ? cvalid >> 2 : cvalid;
+        const float inv = 1.0f / (float)per;
+        const int sh = (per & (per - 1)) ? -1 : __builtin_ctz(per);
+        for (int e = lane; e < 32 * per; e += 64) {
+            const int row = sh >= 0 ? e >> sh : idiv(e, per, inv), k = e - row * per;
+            const int64_t o = orow[row];
+            if (o < 0) continue;
+            const float *src = cbuf + row * RL;
+            if (vec_f)
+                *reinterpret_cast<float4 *>(Yf + o + 4 * k) =
+                    float4{src[4 * k], src[4 * k + 1], src[4 * k + 2], src[4 * k + 3]};
+            else
+                Yf[o + k] = src[k];
+        }
     }
 }
 
@@ -256,6 +357,12 @@ static void conv_tile(int Ds, int H, int W, int R, ConvTile &T)
     T.ntx = (W + T.tx - 1) / T.tx;
     T.nty = (H + T.ty - 1) / T.ty;
     T.Ds = Ds;
+    T.nblk = 0;
+    const int hx = T.tx + 2 * R, hy = T.ty + 2 * R;
+    T.inv_hx = 1.0f / (float)hx;
+    T.inv_hyx = 1.0f / (float)(hx * hy);
+    T.inv_tx = 1.0f / (float)T.tx;
+    T.inv_txy = 1.0f / (float)(T.tx * T.ty);
 }
 
 // Weight gradient of one tap: dW[t][co][ci] += sum_v dY[v][co] * X[v + o_t][ci]
@@ -672,33 +779,36 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 using namespace srf;
 
-extern "C" {
-
-int srf_cnn_conv_batch(const void *x, const void *w, const float *bias, int32_t N, int32_t D, int32_t H, int32_t W,
+static int conv_launch(ConvIn in, const void *w, const float *bias, int32_t N, int32_t D, int32_t H, int32_t W,
                        int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, int32_t out_stride, int32_t out_off,
                        float slope, void *y_bf16, float *y_f32, void *stream)
 {
-    if (!x || !w || N <= 0 || D <= 0 || H <= 0 || W <= 0) return SRF_ERR_INVALID_ARG;
+    if (!in.fine || !w || N <= 0 || D <= 0 || H <= 0 || W <= 0) return SRF_ERR_INVALID_ARG;
     if (cin_p <= 0 || cin_p % 8 || cout <= 0 || cout_p % 32 || cout > cout_p) return SRF_ERR_INVALID_ARG;
     if (taps != 1 && taps != 27) return SRF_ERR_INVALID_ARG;
     if (out_off < 0 || out_off + cout > out_stride || (!y_bf16 && !y_f32)) return SRF_ERR_INVALID_ARG;
-    if ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(w)) & 15) return SRF_ERR_INVALID_ARG;
+    if ((reinterpret_cast<uintptr_t>(in.fine) | reinterpret_cast<uintptr_t>(in.coarse) |
+         reinterpret_cast<uintptr_t>(w)) & 15)
+        return SRF_ERR_INVALID_ARG;
     if ((int64_t)N * D * H * W * (cin_p > out_stride ? cin_p : out_stride) >= ((int64_t)1 << 31))
         return SRF_ERR_UNSUPPORTED;
     ConvDims d{N * D, H, W, cin_p, cout, cout_p, out_stride, out_off, taps, slope};
     const int R = taps == 27 ? 1 : 0;
     ConvTile T;
     conv_tile(D, H, W, R, T);
+    T.nblk = T.ntx * T.nty * (N * D / T.tz);
     const int nb = (cout + 15) / 16;
     const int nbw = (nb >= 4 && cout_p % 64 == 0) ? 4 : nb >= 2 ? 2 : 1;
     const int lg = cin_p >= 32 ? 2 : cin_p >= 16 ? 1 : 0;
-    const size_t lds = (size_t)(T.tx + 2 * R) * (T.ty + 2 * R) * (T.tz + 2 * R) * ((1 << lg) | 1) * 16;
-    dim3 grid((unsigned)((int64_t)T.ntx * T.nty * (N * D / T.tz)), (unsigned)((nb + nbw - 1) / nbw));
-    auto *xs = (const unsigned short *)x;
+    size_t lds = (size_t)(T.tx + 2 * R) * (T.ty + 2 * R) * (T.tz + 2 * R) * ((1 << lg) | 1) * 16;
+    const size_t lds_epi = (size_t)4 * (32 * (16 * nbw + 1) + 64) * sizeof(float);
+    if (lds < lds_epi) lds = lds_epi;
+    // a multiple of 8 workgroups (the XCD-contiguous tile order; extra ones exit)
+    dim3 grid((unsigned)((T.nblk + 7) / 8 * 8), (unsigned)((nb + nbw - 1) / nbw));
     auto *ws = (const unsigned short *)w;
     auto *yb = (unsigned short *)y_bf16;
     auto st = (hipStream_t)stream;
-#define SRF_CONV(TP, NBW) hipLaunchKernelGGL((k_conv<TP, NBW>), grid, dim3(256), lds, st, xs, ws, bias, d, T, yb, y_f32)
+#define SRF_CONV(TP, NBW) hipLaunchKernelGGL((k_conv<TP, NBW>), grid, dim3(256), lds, st, in, ws, bias, d, T, yb, y_f32)
     if (taps == 27) {
         if (nbw == 4) SRF_CONV(27, 4);
         else if (nbw == 2) SRF_CONV(27, 2);
@@ -710,6 +820,26 @@ int srf_cnn_conv_batch(const void *x, const void *w, const float *bias, int32_t 
     }
 #undef SRF_CONV
     return launch_status();
+}
+
+extern "C" {
+
+int srf_cnn_conv_batch(const void *x, const void *w, const float *bias, int32_t N, int32_t D, int32_t H, int32_t W,
+                       int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, int32_t out_stride, int32_t out_off,
+                       float slope, void *y_bf16, float *y_f32, void *stream)
+{
+    return conv_launch(ConvIn{(const unsigned short *)x, nullptr, 0}, w, bias, N, D, H, W, cin_p, cout, cout_p, taps,
+                       out_stride, out_off, slope, y_bf16, y_f32, stream);
+}
+
+int srf_cnn_conv_up_batch(const void *fine, const void *coarse, int32_t cc, const void *w, const float *bias,
+                          int32_t N, int32_t D, int32_t H, int32_t W, int32_t cin_p, int32_t cout, int32_t cout_p,
+                          int32_t taps, int32_t out_stride, int32_t out_off, float slope, void *y_bf16, float *y_f32,
+                          void *stream)
+{
+    if (!coarse || cc <= 0 || cc % 8 || cc >= cin_p || D % 2 || H % 2 || W % 2) return SRF_ERR_INVALID_ARG;
+    return conv_launch(ConvIn{(const unsigned short *)fine, (const unsigned short *)coarse, cc}, w, bias, N, D, H, W,
+                       cin_p, cout, cout_p, taps, out_stride, out_off, slope, y_bf16, y_f32, stream);
 }
 
 int srf_cnn_conv(const void *x, const void *w, const float *bias, int32_t D, int32_t H, int32_t W, int32_t cin_p,

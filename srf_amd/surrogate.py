@@ -77,6 +77,8 @@ class UNet:
     def __init__(self, in_ch=4, widths=(16, 32, 64, 128), slope=SLOPE, seed=0, device=None):
         self.dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.in_ch, self.widths, self.slope = in_ch, tuple(widths), float(slope)
+        if any(int(v) % 8 for v in self.widths):
+            raise ValueError(f"widths {self.widths} must be multiples of 8 (the decoder reads its skip input in place)")
         self.layers = [_Layer(n, t, ci, co, self.dev) for n, t, ci, co in unet_layers(in_ch, widths)]
         self.L = {l.name: l for l in self.layers}
         self.step_count = 0
@@ -164,7 +166,6 @@ class UNet:
             B[f"ea{l}"], B[f"e{l}"] = bf(V[l], w[l]), bf(V[l], w[l])
             if l + 1 < n:
                 B[f"p{l}"] = bf(V[l + 1], w[l])
-                B[f"u{l}"] = bf(V[l], w[l + 1] + w[l])
                 B[f"da{l}"], B[f"d{l}"] = bf(V[l], w[l]), bf(V[l], w[l])
         B["out"] = f32(V[0])
         self._bufs[key] = B
@@ -185,6 +186,7 @@ class UNet:
             B[f"dp{l}"] = bf(V[l], max(cmax, 8))
             if l + 1 < n:
                 B[f"gp{l}"] = f32(V[l + 1], w[l])
+                B[f"u{l}"] = bf(V[l], w[l + 1] + w[l])      # the decoder input, for its weight gradient
                 B[f"gu{l}"] = f32(V[l], w[l + 1] + w[l])
                 B[f"gda{l}"], B[f"gd{l}"] = f32(V[l], w[l]), f32(V[l], w[l])
         B["gout"] = f32(V[0])
@@ -240,13 +242,13 @@ class UNet:
             self._conv(self.L[f"enc{l}b"], B[f"ea{l}"], d, N, y_bf=B[f"e{l}"])
         prev = B[f"e{n - 1}"]
         for l in range(n - 2, -1, -1):
-            d, cd = B["dims"][l], B["dims"][l + 1]
-            cu = w[l + 1] + w[l]
-            check(lib().srf_cnn_upsample(prev.data_ptr(), N * cd[0], cd[1], cd[2], w[l + 1], w[l + 1],
-                                         B[f"u{l}"].data_ptr(), cu, 0, s), "srf_cnn_upsample")
-            check(lib().srf_cnn_copy_channels(B[f"e{l}"].data_ptr(), B["V"][l], w[l], w[l], 0, B[f"u{l}"].data_ptr(),
-                                              cu, w[l + 1], 0, 0, s), "srf_cnn_copy_channels")
-            self._conv(self.L[f"dec{l}a"], B[f"u{l}"], d, N, y_bf=B[f"da{l}"])
+            d = B["dims"][l]
+            # dec_a reads [upsample(prev), e_l] in place (no concatenated copy)
+            L = self.L[f"dec{l}a"]
+            check(lib().srf_cnn_conv_up_batch(B[f"e{l}"].data_ptr(), prev.data_ptr(), w[l + 1], L.wbf.data_ptr(),
+                                              L.b.data_ptr(), N, *d, L.cin_p, L.cout, L.cout_p, L.taps, L.cout, 0,
+                                              self.slope, B[f"da{l}"].data_ptr(), None, s),
+                  "srf_cnn_conv_up_batch " + L.name)
             self._conv(self.L[f"dec{l}b"], B[f"da{l}"], d, N, y_bf=B[f"d{l}"])
             prev = B[f"d{l}"]
         self._conv(self.L["head"], prev, B["dims"][0], N, y_f32=B["out"], stride=1, act=False)
@@ -266,6 +268,12 @@ class UNet:
         for l in range(0, n - 1):
             d, cd = B["dims"][l], B["dims"][l + 1]
             cu = w[l + 1] + w[l]
+            # the decoder input the forward read in place, materialised for dW
+            coarse = B[f"d{l + 1}"] if l + 1 < n - 1 else B[f"e{n - 1}"]
+            check(lib().srf_cnn_upsample(coarse.data_ptr(), N * cd[0], cd[1], cd[2], w[l + 1], w[l + 1],
+                                         B[f"u{l}"].data_ptr(), cu, 0, s), "srf_cnn_upsample")
+            check(lib().srf_cnn_copy_channels(B[f"e{l}"].data_ptr(), B["V"][l], w[l], w[l], 0, B[f"u{l}"].data_ptr(),
+                                              cu, w[l + 1], 0, 0, s), "srf_cnn_copy_channels")
             self._back(self.L[f"dec{l}b"], B[f"da{l}"], B[f"d{l}"], B[f"gd{l}"], w[l], d, N, dp(l),
                        gx=B[f"gda{l}"])
             self._back(self.L[f"dec{l}a"], B[f"u{l}"], B[f"da{l}"], B[f"gda{l}"], w[l], d, N, dp(l),

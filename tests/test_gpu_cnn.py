@@ -168,6 +168,36 @@ def test_batched_conv_and_gradients_per_sample(dims, cin, cout):
     assert _rel(db.double().cpu().numpy(), want_b) <= 1e-5
 
 
+def test_conv_reads_upsampled_concatenation_in_place():
+    """srf_cnn_conv_up_batch = the convolution of [upsample2x(coarse), fine]
+    (two samples; 32 coarse + 16 fine channels, a chunk from each input)."""
+    from srf_amd.surrogate import _c32
+    rng = np.random.default_rng(21)
+    N, dims, cc, cf, cout, taps = 2, (4, 6, 16), 32, 16, 24, 27
+    cdims = tuple(v // 2 for v in dims)
+    cin_p, cout_p = cc + cf, _c32(cout)
+    V = int(np.prod(dims))
+    fd, fr = _bf(rng.normal(size=(N,) + dims + (cf,)))
+    cd_, cr = _bf(rng.normal(size=(N,) + cdims + (cc,)))
+    w = rng.normal(size=(taps, cout, cin_p)) / np.sqrt(taps * cin_p)
+    wd, wr = _bf(_pad_w(w, cout_p, cin_p))
+    b = rng.normal(size=cout).astype(np.float32)
+    y = torch.zeros((N * V, cout), dtype=torch.float32, device=DEV)
+    print(f"conv of upsampled concatenation N={N} dims={dims}", flush=True)
+    assert _L().srf_cnn_conv_up_batch(fd.data_ptr(), cd_.data_ptr(), cc, wd.data_ptr(),
+                                      torch.from_numpy(b).to(DEV).data_ptr(), N, *dims, cin_p, cout, cout_p, taps,
+                                      cout, 0, 0.01, None, y.data_ptr(), _s()) == 0
+    # a coarse channel count that is not a multiple of 8 is refused
+    assert _L().srf_cnn_conv_up_batch(fd.data_ptr(), cd_.data_ptr(), 12, wd.data_ptr(), None, N, *dims, cin_p, cout,
+                                      cout_p, taps, cout, 0, 0.01, None, y.data_ptr(), _s()) != 0
+    torch.cuda.synchronize()
+    got = y.double().cpu().numpy().reshape(N, V, cout)
+    for i in range(N):
+        u = np.concatenate([co.upsample(cr[i]), fr[i]], -1)
+        want = co.leaky(co.conv3d(u, wr[:, :cout, :], b.astype(np.float64)), 0.01).reshape(V, cout)
+        assert _rel(got[i], want) <= 1e-5, i
+
+
 def test_unet_batch_equals_per_sample():
     """The U-Net over three stacked samples: outputs bit-identical to three
     single-sample passes, parameter gradients equal to their sum (1e-5)."""
