@@ -364,6 +364,14 @@ int srf_cnn_conv_up_batch(const void *fine, const void *coarse, int32_t cc, cons
                           int32_t N, int32_t D, int32_t H, int32_t W, int32_t cin_p, int32_t cout, int32_t cout_p,
                           int32_t taps, int32_t out_stride, int32_t out_off, float slope, void *y_bf16, float *y_f32,
                           void *stream);
+/* srf_cnn_conv_batch into y_bf16 (rows of cout channels) followed by a 1x1
+ * convolution to one channel in the same pass: head_out[v] = head_b[0] +
+ * sum_n y_bf16[v][n] head_w[n] (head_w: cout bf16 weights; float sums, so
+ * equal to srf_cnn_conv of y_bf16 with taps 1 up to float rounding).  cout <=
+ * 64 with every channel in one workgroup; SRF_ERR_UNSUPPORTED otherwise. */
+int srf_cnn_conv_head_batch(const void *x, const void *w, const float *bias, int32_t N, int32_t D, int32_t H,
+                            int32_t W, int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, float slope,
+                            void *y_bf16, const void *head_w, const float *head_b, float *head_out, void *stream);
 /* dw[t][n][c] += sum_v dy[v][n] x[v + o_t][c]   (dy bf16 rows of dy_stride);
  * the _batch form over N stacked samples, and with db != NULL also
  * db[n] += sum_v dy[v][n] (the bias gradient, as srf_cnn_bias_grad). */

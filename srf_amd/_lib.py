@@ -129,6 +129,8 @@ SIGNATURES = [
                                           _F, _P, _P, _P]),
     ("srf_cnn_conv_up_batch", ctypes.c_int, [_P, _P, _I32, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32,
                                              _I32, _I32, _F, _P, _P, _P]),
+    ("srf_cnn_conv_head_batch", ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _F, _P,
+                                               _P, _P, _P, _P]),
     ("srf_cnn_conv_wgrad_batch", ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P,
                                                 _P, _P]),
     ("srf_cnn_flip_weights", ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I32, _I32, _P]),

@@ -93,6 +93,15 @@ struct ConvIn {
     int cc;
 };
 
+// A 1x1 convolution to one channel applied to a layer's output in its
+// epilogue: out[v] = b + sum_c bf16(y[v][c]) w[c] (the bf16 values the layer
+// stores), so that the network's last layer needs no pass of its own.
+struct ConvHead {
+    const unsigned short *w;   // cout bf16 weights
+    const float *b;            // the bias (one value)
+    float *out;                // one value per voxel
+};
+
 // floor(a / b) for 0 <= a < 2^24 and b >= 1 given inv = 1/b in float: the
 // float quotient is off by at most one, and one integer step corrects it.
 __device__ __forceinline__ int idiv(int a, int b, float inv)
@@ -165,7 +174,7 @@ __device__ __forceinline__ void conv_chunk(const u16x8 *__restrict__ halo, const
 template <int TAPS, int NBW>
 __global__ __launch_bounds__(256) void k_conv(ConvIn in, const unsigned short *__restrict__ Wt,
                                               const float *__restrict__ bias, ConvDims d, ConvTile T,
-                                              unsigned short *__restrict__ Ybf, float *__restrict__ Yf)
+                                              unsigned short *__restrict__ Ybf, float *__restrict__ Yf, ConvHead hd)
 {
     constexpr int R = TAPS == 27 ? 1 : 0;
     constexpr int STAGE = 4;   // halo loads in flight per thread
@@ -282,16 +291,23 @@ __global__ __launch_bounds__(256) void k_conv(ConvIn in, const unsigned short *_
             }
     }
     // global offset of each row of this wavefront, -1 outside the volume
+    int myvox = -1;
     if (lane < 32) {
         const int tv = wave * 32 + lane;
         const int ltz = idiv(tv, txy, T.inv_txy), rem = tv - ltz * txy;
         const int lty = idiv(rem, T.tx, T.inv_tx), ltx = rem - lty * T.tx;
         const int gx = x0 + ltx, gy = y0 + lty, gz = z0 + ltz;
-        orow[lane] = (tv < nvox && gx < d.W && gy < d.H)
-                         ? ((int64_t)(gz * d.H + gy) * d.W + gx) * d.out_stride + d.out_off + n0
-                         : -1;
+        if (tv < nvox && gx < d.W && gy < d.H) myvox = (gz * d.H + gy) * d.W + gx;
+        orow[lane] = myvox >= 0 ? (int64_t)myvox * d.out_stride + d.out_off + n0 : -1;
     }
     __syncthreads();   // LDS writes visible to the whole wavefront
+    if (hd.out && myvox >= 0) {
+        // the head on this lane's row: products of bf16 values are exact in float
+        const float *src = cbuf + lane * RL;
+        float sum = 0.0f;
+        for (int c = 0; c < d.cout; ++c) sum += bf2f(f2bf(src[c])) * bf2f(hd.w[c]);
+        hd.out[myvox] = sum + hd.b[0];
+    }
     const int cvalid = min(16 * NBW, d.cout - n0);   // channels of this block to store
     const bool vec_bf = Ybf && (cvalid % 8 == 0) && ((d.out_stride | d.out_off | n0) % 8 == 0) &&
                         !(reinterpret_cast<uintptr_t>(Ybf) & 15);
@@ -781,7 +797,8 @@ using namespace srf;
 
 static int conv_launch(ConvIn in, const void *w, const float *bias, int32_t N, int32_t D, int32_t H, int32_t W,
                        int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, int32_t out_stride, int32_t out_off,
-                       float slope, void *y_bf16, float *y_f32, void *stream)
+                       float slope, void *y_bf16, float *y_f32, void *stream, ConvHead hd = ConvHead{nullptr, nullptr,
+                                                                                                     nullptr})
 {
     if (!in.fine || !w || N <= 0 || D <= 0 || H <= 0 || W <= 0) return SRF_ERR_INVALID_ARG;
     if (cin_p <= 0 || cin_p % 8 || cout <= 0 || cout_p % 32 || cout > cout_p) return SRF_ERR_INVALID_ARG;
@@ -805,10 +822,12 @@ static int conv_launch(ConvIn in, const void *w, const float *bias, int32_t N, i
     if (lds < lds_epi) lds = lds_epi;
     // a multiple of 8 workgroups (the XCD-contiguous tile order; extra ones exit)
     dim3 grid((unsigned)((T.nblk + 7) / 8 * 8), (unsigned)((nb + nbw - 1) / nbw));
+    if (hd.out && (grid.y != 1 || !hd.w || !hd.b)) return SRF_ERR_UNSUPPORTED;   // the head needs every channel
     auto *ws = (const unsigned short *)w;
     auto *yb = (unsigned short *)y_bf16;
     auto st = (hipStream_t)stream;
-#define SRF_CONV(TP, NBW) hipLaunchKernelGGL((k_conv<TP, NBW>), grid, dim3(256), lds, st, in, ws, bias, d, T, yb, y_f32)
+#define SRF_CONV(TP, NBW) \
+    hipLaunchKernelGGL((k_conv<TP, NBW>), grid, dim3(256), lds, st, in, ws, bias, d, T, yb, y_f32, hd)
     if (taps == 27) {
         if (nbw == 4) SRF_CONV(27, 4);
         else if (nbw == 2) SRF_CONV(27, 2);
@@ -830,6 +849,16 @@ int srf_cnn_conv_batch(const void *x, const void *w, const float *bias, int32_t 
 {
     return conv_launch(ConvIn{(const unsigned short *)x, nullptr, 0}, w, bias, N, D, H, W, cin_p, cout, cout_p, taps,
                        out_stride, out_off, slope, y_bf16, y_f32, stream);
+}
+
+int srf_cnn_conv_head_batch(const void *x, const void *w, const float *bias, int32_t N, int32_t D, int32_t H,
+                            int32_t W, int32_t cin_p, int32_t cout, int32_t cout_p, int32_t taps, float slope,
+                            void *y_bf16, const void *head_w, const float *head_b, float *head_out, void *stream)
+{
+    if (!y_bf16 || !head_out || !head_w || !head_b) return SRF_ERR_INVALID_ARG;
+    return conv_launch(ConvIn{(const unsigned short *)x, nullptr, 0}, w, bias, N, D, H, W, cin_p, cout, cout_p, taps,
+                       cout, 0, slope, y_bf16, nullptr, stream,
+                       ConvHead{(const unsigned short *)head_w, head_b, head_out});
 }
 
 int srf_cnn_conv_up_batch(const void *fine, const void *coarse, int32_t cc, const void *w, const float *bias,

@@ -249,7 +249,17 @@ class UNet:
                                               L.b.data_ptr(), N, *d, L.cin_p, L.cout, L.cout_p, L.taps, L.cout, 0,
                                               self.slope, B[f"da{l}"].data_ptr(), None, s),
                   "srf_cnn_conv_up_batch " + L.name)
-            self._conv(self.L[f"dec{l}b"], B[f"da{l}"], d, N, y_bf=B[f"d{l}"])
+            Lb = self.L[f"dec{l}b"]
+            if l == 0 and Lb.cout <= 64:
+                # the 1x1 head in dec0b's epilogue
+                H_ = self.L["head"]
+                check(lib().srf_cnn_conv_head_batch(B["da0"].data_ptr(), Lb.wbf.data_ptr(), Lb.b.data_ptr(), N, *d,
+                                                    Lb.cin_p, Lb.cout, Lb.cout_p, Lb.taps, self.slope,
+                                                    B["d0"].data_ptr(), H_.wbf.data_ptr(), H_.b.data_ptr(),
+                                                    B["out"].data_ptr(), s), "srf_cnn_conv_head_batch " + Lb.name)
+                self._x0 = x0
+                return B["out"]
+            self._conv(Lb, B[f"da{l}"], d, N, y_bf=B[f"d{l}"])
             prev = B[f"d{l}"]
         self._conv(self.L["head"], prev, B["dims"][0], N, y_f32=B["out"], stride=1, act=False)
         self._x0 = x0

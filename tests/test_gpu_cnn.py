@@ -198,6 +198,42 @@ def test_conv_reads_upsampled_concatenation_in_place():
         assert _rel(got[i], want) <= 1e-5, i
 
 
+def test_conv_with_head_in_the_epilogue():
+    """srf_cnn_conv_head_batch: the layer's bf16 output as srf_cnn_conv_batch
+    writes it, and the 1x1 one-channel head of that output within float
+    rounding of the separate head convolution (and of the oracle)."""
+    from srf_amd.surrogate import _c8, _c32
+    rng = np.random.default_rng(23)
+    N, dims, cin, cout, taps = 2, (4, 8, 32), 16, 16, 27
+    cin_p, cout_p = _c8(cin), _c32(cout)
+    V = int(np.prod(dims))
+    xd, xr = _bf(rng.normal(size=(N,) + dims + (cin_p,)))
+    w = rng.normal(size=(taps, cout, cin)) / np.sqrt(taps * cin)
+    wd, wr = _bf(_pad_w(w, cout_p, cin_p))
+    b = torch.from_numpy(rng.normal(size=cout).astype(np.float32)).to(DEV)
+    hw = rng.normal(size=(1, 1, cout))
+    hwd, hwr = _bf(_pad_w(hw, 32, cout))
+    hb = torch.tensor([0.3], dtype=torch.float32, device=DEV)
+    y1 = torch.zeros((N * V, cout), dtype=torch.int16, device=DEV)
+    y2 = torch.zeros_like(y1)
+    out = torch.zeros(N * V, dtype=torch.float32, device=DEV)
+    ref = torch.zeros(N * V, dtype=torch.float32, device=DEV)
+    print(f"conv + head epilogue N={N} dims={dims}", flush=True)
+    assert _L().srf_cnn_conv_batch(xd.data_ptr(), wd.data_ptr(), b.data_ptr(), N, *dims, cin_p, cout, cout_p, taps,
+                                   cout, 0, 0.01, y1.data_ptr(), None, _s()) == 0
+    assert _L().srf_cnn_conv_head_batch(xd.data_ptr(), wd.data_ptr(), b.data_ptr(), N, *dims, cin_p, cout, cout_p,
+                                        taps, 0.01, y2.data_ptr(), hwd.data_ptr(), hb.data_ptr(), out.data_ptr(),
+                                        _s()) == 0
+    assert _L().srf_cnn_conv_batch(y1.data_ptr(), hwd.data_ptr(), hb.data_ptr(), N, *dims, cout, 1, 32, 1, 1, 0,
+                                   -1.0, None, ref.data_ptr(), _s()) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    got, want = out.double().cpu().numpy(), ref.double().cpu().numpy()
+    assert _rel(got, want) <= 1e-6
+    orc = (_f(y1).reshape(N * V, cout) @ hwr[0, 0, :cout]) + 0.3
+    assert _rel(got, orc) <= 1e-6
+
+
 def test_unet_batch_equals_per_sample():
     """The U-Net over three stacked samples: outputs bit-identical to three
     single-sample passes, parameter gradients equal to their sum (1e-5)."""
