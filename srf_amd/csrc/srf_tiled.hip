@@ -283,7 +283,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_count(const float *__restri
 template <bool TOF>
 __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restrict__ lors, int64_t n, int stride,
                                                           const float *__restrict__ tofv, int tof_stride,
-                                                          LaunchGeom G, TileCfg T, float twin,
+                                                          LaunchGeom G, TileCfg T, float twin, int prescale,
                                                           const int *__restrict__ ev_off,
                                                           SegRec *__restrict__ recs, unsigned *__restrict__ rec_sk)
 {
@@ -297,10 +297,15 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
         const int pos = j++;
         SegRec R;
         R.p1 = B.p1;
-        R.sec1 = B.sec1;
-        R.ks = B.ks;
-        R.z1 = r.z1;
-        R.kz = B.kz;
+        // prescale bit 0 / 1: the secondary / axial voxel edge is a power of
+        // two, and the record holds the line's constants divided by it (exact):
+        // the marcher's (c + k (p - p1)) / d then equals c / d + (k / d) (p - p1)
+        // bit for bit and runs without the division
+        const float ss = (prescale & 1) ? 1.0f / B.ds : 1.0f, zs = (prescale & 2) ? 1.0f / G.dz : 1.0f;
+        R.sec1 = B.sec1 * ss;
+        R.ks = B.ks * ss;
+        R.z1 = r.z1 * zs;
+        R.kz = B.kz * zs;
         R.Wbp = branch_weight<TOF>(B, r, G, false);
         R.Wfp = branch_weight<TOF>(B, r, G, true);
         tof_coeffs<TOF>(B, r, G, R.tA, R.tB);
@@ -1465,6 +1470,14 @@ namespace {
 
 // Bins an event list into P (whose buffers are grown as needed and otherwise
 // reused); on failure P holds no valid list.
+// v = 2^k with 2^-64 <= v <= 2^64, k != 0: dividing by v is multiplying by
+// the exact 1 / v (no subnormal for the coordinates and slopes involved).
+static bool pow2_edge(float v)
+{
+    int e = 0;
+    return v != 1.0f && v >= 0x1p-64f && v <= 0x1p64f && frexpf(v, &e) == 0.5f;
+}
+
 int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, const float *tof_values,
                int64_t tof_stride, srf_grid_t grid, const srf_tof_t *tof, const srf_plan_opts_t *opts,
                hipStream_t s)
@@ -1482,6 +1495,7 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     int rc = make_geom(grid, tof, G);
     if (rc) return rc;
     if (grid.nx > 65535 || grid.ny > 65535) return SRF_ERR_UNSUPPORTED;   // slab index packs in 16 bits
+    const int prescale = (pow2_edge(G.dy) ? 1 : 0) | (pow2_edge(G.dz) ? 2 : 0);
     TileCfg T;
     T.ntx = (grid.nx + TX - 1) / TX;
     T.nty = (grid.ny + TY - 1) / TY;
@@ -1579,10 +1593,10 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
         PB_TRY(ensure(P->wpair, (size_t)nseg, s));
         if (use_tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->sk.p);
+                               tof_values, (int)tof_stride, G, T, twin, prescale, P->ev_off.p, P->recs.p, P->sk.p);
         else
             hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, P->ev_off.p, P->recs.p, P->sk.p);
+                               tof_values, (int)tof_stride, G, T, twin, prescale, P->ev_off.p, P->recs.p, P->sk.p);
         // Record order: a stable sort of the slot-ordered records by (key,
         // branch, slab count) -- within one, slot order.  The order (and with
         // it the split into work items and each item's fixed-point scale) is a
@@ -1614,8 +1628,8 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     P->G = G;
     P->T = T;
     P->tof = use_tof;
-    P->ds = G.dy != 1.0f;
-    P->dz = G.dz != 1.0f;
+    P->ds = G.dy != 1.0f && !(prescale & 1);
+    P->dz = G.dz != 1.0f && !(prescale & 2);
     P->n = n;
     P->nseg = nseg;
     P->nitems = nitems;

@@ -296,4 +296,69 @@ def reconstruct_batched(scans, n_sub=8, n_iter=4, estimator=None, config=BatchCo
     return imgs, t, info
 
 
+def _main(argv=None):
+    """Time scatter estimates on the batched workload's scans: per scan the
+    plain SSS (its steps separately: scatter points and attenuation
+    integrals, emission line integrals, the pair and bin kernels), then one
+    surrogate estimate of all the scans together.  Each estimate runs twice;
+    the second (warm: cached attenuation integrals, built plans) is timed."""
+    import argparse
+    import os
+    from ._lib import check, lib, stream_handle
+    from .surrogate import UNet
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scans", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=5)
+    a = ap.parse_args(argv)
+    dev = torch.device("cuda", 0)
+    scans = make_scans(n_scans=32, seed=a.seed, device=dev)[:a.scans]
+    here = os.path.dirname(os.path.abspath(__file__))
+    net, meta = UNet.load(os.path.join(here, "weights", "scatter_unet_in_loop.npz"), device=dev)
+    est = ScatterEstimator(net=net, kappa=meta["kappa"], coarse_spacing=meta.get("coarse_spacing") or None)
+    xs = [torch.full(tuple(int(v) for v in sc.shape), 0.5, device=dev) for sc in scans]
+    T = {}
+
+    def tick(k, t0):
+        torch.cuda.synchronize()
+        T[k] = T.get(k, 0.0) + time.time() - t0
+        return time.time()
+
+    for rep in range(2):
+        T.clear()
+        for sc, x in zip(scans, xs):
+            torch.cuda.synchronize()
+            t = time.time()
+            xb = body_estimate(x, sc.meta["mu"], sc.meta.get("voxel"))
+            k = est._key(sc)
+            if k not in est._sss:
+                est._sss[k] = SingleScatterSimulation(est.g, sc.shape, sc.center, sc.size, cell_normalized=True)
+            s = est._sss[k]
+            t = tick("body estimate (smoothing)", t)
+            pts, mu = s._mu_part(sc.meta["mu"])
+            t = tick("sss: scatter points + attenuation integrals (cached after the first)", t)
+            em = s.line_integrals(xb, pts)
+            t = tick("sss: emission line integrals", t)
+            rows = s._rows_for(pts.shape[0])
+            st = stream_handle(dev)
+            check(lib().srf_sss_prepare(s.det.data_ptr(), s.nd, pts.data_ptr(), pts.shape[0], em.data_ptr(),
+                                        mu.data_ptr(), rows.data_ptr(), rows.numel() * 4, st), "srf_sss_prepare")
+            t = tick("sss: srf_sss_prepare (k_sss_pairs)", t)
+            out = torch.zeros(s.bins.shape[0], device=dev)
+            check(lib().srf_sss_sinogram(rows.data_ptr(), s.nd, pts.data_ptr(), pts.shape[0], s.bins.data_ptr(),
+                                         s.bins.shape[0], s.tables.data_ptr(), float(s.scale), out.data_ptr(), st),
+                  "srf_sss_sinogram")
+            t = tick("sss: srf_sss_sinogram (k_sss_bins)", t)
+        torch.cuda.synchronize()
+        t = time.time()
+        est.surrogate_batch([(i, sc, x) for i, (sc, x) in enumerate(zip(scans, xs))])
+        tick(f"surrogate: all {len(scans)} scans in one batch (features, check ring, CNN)", t)
+    for k, v in T.items():
+        per = v * 1e3 / (1 if k.startswith("surrogate") else len(scans))
+        print(f"{k:75s} {per:9.2f} ms{'' if k.startswith('surrogate') else ' per scan'}", flush=True)
+
+
+if __name__ == "__main__":
+    _main()
+
+
 __all__ = ["make_scans", "ScatterEstimator", "reconstruct_plain", "reconstruct_batched"]

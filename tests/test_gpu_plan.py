@@ -67,6 +67,38 @@ def test_plan_digest_and_values_vs_golden(golden, gname, tof):
 
 
 @pytest.mark.parametrize("tof", [False, True])
+def test_plan_power_of_two_voxel_ratios_vs_oracle(golden, orc, tof):
+    """dy = dz = 2 dx: the plan divides the line constants by the voxel ratio
+    once at build time instead of dividing each slab's coordinates by it (an
+    exact rewrite for powers of two).  Term digests equal the oracle's bit for
+    bit; projection and both backprojections within 1e-5."""
+    import torch
+    from srf_amd import ops
+    shape, center, size = (32, 16, 16), [0.5, -0.25, 0.0], [64.0, 64.0, 64.0]
+    lors = golden["lors"] if tof else np.ascontiguousarray(golden["lors"][:, :6])
+    t = TOF if tof else None
+    tb, ts = TOF if tof else (0.0, 0.0)
+    L = torch.from_numpy(np.ascontiguousarray(lors)).cuda()
+    print(f"plan dy = dz = 2 dx tof={tof}", flush=True)
+    plan = ops.Plan(L, shape, list(center) + list(size), tof, tb, ts)
+    for mode, var in ((1, 0), (0, 1)):
+        h, c = plan.digest(mode)
+        torch.cuda.synchronize()
+        wh, wc = orc.digest(lors, shape, center, size, tof=t, variant=var)
+        bad = np.nonzero((h.cpu().numpy().view(np.uint64) != wh) | (c.cpu().numpy() != wc))[0]
+        assert len(bad) == 0, f"mode {mode}: {len(bad)} events differ, first {bad[:10]}"
+    rng = np.random.default_rng(3)
+    img = rng.uniform(0.1, 2.0, shape).astype(np.float32)
+    vals = rng.uniform(0.5, 1.5, len(lors)).astype(np.float32)
+    p = plan.project(ops.to_kernel_layout(torch.from_numpy(img).cuda())).cpu().numpy()
+    assert rel_l2(p, orc.project(img, lors, center, size, tof=t, f64=True)) <= 1e-5
+    for mode, var in ((0, 1), (1, 0)):
+        bp = ops.from_kernel_layout(plan.backproject(torch.from_numpy(vals).cuda(), mode)).cpu().numpy()
+        assert rel_l2(bp, orc.backproject(vals, lors, shape, center, size, tof=t, variant=var)) <= 1e-5
+    plan.close()
+
+
+@pytest.mark.parametrize("tof", [False, True])
 def test_plan_matches_event_kernels_scanner_size(tof):
     """cfg-2 geometry (256x256x128, 2 mm, R 424 mm): tiled vs one-event-per-lane."""
     import torch
