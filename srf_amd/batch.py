@@ -19,7 +19,7 @@ import torch
 
 from . import ops
 from ._lib import SrfBatchCfg, SrfBatchScan, check, lib, stream_handle
-from .functions import image_from_kernel, image_to_kernel
+from .functions import geometric_subset_order, image_from_kernel, image_to_kernel
 
 
 def scan_permutation(n, seed, device):
@@ -73,7 +73,8 @@ class BatchOsem:
             n = sc.lors.shape[0]
             perm = scan_permutation(n, sc.seed, self.dev)
             nps = n // self.n_sub
-            pidx = perm[: nps * self.n_sub]
+            # OsemEngine's subsets: the same permutation and in-subset order
+            pidx = geometric_subset_order(sc.lors, perm[: nps * self.n_sub], self.n_sub)
             self.lors.append(sc.lors.index_select(0, pidx).contiguous())
             vals = sc.values if sc.values is not None else torch.ones(n, dtype=torch.float32, device=self.dev)
             self.y.append(vals.reshape(-1).to(self.dev, torch.float32).index_select(0, pidx).contiguous())
@@ -81,7 +82,7 @@ class BatchOsem:
             nx, ny, nz = [int(v) for v in sc.shape]
             self.x.append(torch.ones((nz, ny, nx), dtype=torch.float32, device=self.dev))
             self.emap.append(image_to_kernel(sc.emap).contiguous())
-            self.perm.append(perm)
+            self.perm.append(pidx)
             self.n_per_sub.append(nps)
         subsets = range(self.n_sub) if subsets is None else subsets
         self.plans, self.batches = {}, {}
@@ -177,7 +178,8 @@ class BatchOsem:
         """What a checkpoint of this batch must match: every scan's events and
         grid, OSEM constants, TOF and plan options, and the launch schedule (the
         autotuned item size changes how partial sums are grouped)."""
-        return {"kind": "batch_osem", "n_sub": self.n_sub, "ratio_kind": self.ratio_kind, "eps": self.eps,
+        return {"kind": "batch_osem", "event_order": "geometric", "n_sub": self.n_sub,
+                "ratio_kind": self.ratio_kind, "eps": self.eps,
                 "zero_guard": self.zero_guard, "tof": list(self.tof), "config": dict(self.config.__dict__),
                 "scans": [[int(sc.lors.shape[0]), int(sc.seed), [int(v) for v in sc.shape], sc.geom]
                           for sc in self.scans]}
@@ -213,8 +215,7 @@ class BatchOsem:
     def set_values(self, i, values):
         """Replace scan i's event values (original event order) in place; the
         batch reads them at every step, so a captured graph sees the change."""
-        pidx = self.perm[i][: self.n_per_sub[i] * self.n_sub]
-        self.y[i].copy_(values.reshape(-1).to(self.dev, torch.float32).index_select(0, pidx))
+        self.y[i].copy_(values.reshape(-1).to(self.dev, torch.float32).index_select(0, self.perm[i]))
         self.values_set = True
 
     def image(self, i):

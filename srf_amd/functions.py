@@ -271,6 +271,37 @@ class MlemFullTof:
         return _mlem_full(self, getattr(self.scanner, "tof", None) or TofConfig())
 
 
+def geometric_subset_order(lors, pidx, n_sub):
+    """The subsets of ``pidx`` (n_sub contiguous chunks of event indices) with
+    each subset's events ordered by LOR geometry: transaxial angle (128 bins
+    over [0, pi)), axial midpoint (16 mm bins), then signed transaxial
+    distance from the axis (1 mm bins), a stable sort.  Membership, and with
+    it every EM update, is unchanged (the plans' sums do not depend on event
+    order); events that cross the same image tile get nearby indices, so the
+    tiled kernels' per-event reads and writes (ratios, partial sums) touch a
+    few cache lines per tile instead of one per event."""
+    n = int(pidx.numel())
+    if n == 0:
+        return pidx
+    nps = n // int(n_sub)
+    L = lors.index_select(0, pidx)
+    p1, d = L[:, 0:3], L[:, 3:6] - L[:, 0:3]
+    phi = torch.atan2(d[:, 1], d[:, 0])
+    flip = phi < 0
+    phi = torch.where(flip, phi + math.pi, phi)
+    h = torch.hypot(d[:, 0], d[:, 1])
+    r = (p1[:, 0] * d[:, 1] - p1[:, 1] * d[:, 0]) / torch.where(h > 0, h, torch.ones_like(h))
+    r = torch.where(flip, -r, r)
+    zm = 0.5 * (L[:, 2] + L[:, 5])
+    ab = torch.nan_to_num((phi * (128.0 / math.pi)).floor(), 0.0).clamp(0, 127).long()
+    zb = torch.nan_to_num((zm / 16.0).floor() + 512, 0.0).clamp(0, 1023).long()
+    rb = torch.nan_to_num(r.floor() + 32768, 0.0).clamp(0, 65535).long()
+    sub = torch.arange(n, device=pidx.device) // max(nps, 1)
+    key = ((sub * 128 + ab) * 1024 + zb) * 65536 + rb
+    order = torch.sort(key, stable=True).indices
+    return pidx.index_select(0, order)
+
+
 def subset_permutation(n, seed):
     """Seeded event permutation (the reference shuffles with the unseeded
     global RNG, osem_full.py:63-64)."""
@@ -283,7 +314,7 @@ class OsemEngine(EmEngine):
     with ``tiled`` each subset is binned into its own plan at construction."""
 
     def __init__(self, lors_data, values, emap_data, shape, geom, n_sub, perm, tof=None, ratio_kind=0,
-                 eps=1e-8, zero_guard=0.0, tiled=True, item_records=8192, tof_window=0.0):
+                 eps=1e-8, zero_guard=0.0, tiled=True, item_records=8192, tof_window=0.0, geometric_order=True):
         lors = to_gpu(lors_data)
         y = to_gpu(values).reshape(-1)
         n = lors.shape[0]
@@ -294,6 +325,12 @@ class OsemEngine(EmEngine):
         else:
             pidx = torch.as_tensor(np.asarray(perm[: self.n_per_sub * self.n_sub]), device=lors.device,
                                    dtype=torch.long)
+        if geometric_order:
+            pidx = geometric_subset_order(lors, pidx, self.n_sub)
+        self.event_order = "geometric" if geometric_order else "permutation"
+        # event_index[j]: the caller's index of this engine's event j (subset
+        # k is j in [k n_per_sub, (k + 1) n_per_sub))
+        self.event_index = pidx
         super().__init__(lors.index_select(0, pidx).contiguous(), y.index_select(0, pidx).contiguous(),
                          emap_data, shape, geom, tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard,
                          tiled=tiled, plan=False, item_records=item_records, tof_window=tof_window)
@@ -301,7 +338,7 @@ class OsemEngine(EmEngine):
         step = max(1, pidx.numel() // 65536)
         self.perm_tag = f"{int(pidx.numel())}:{int(pidx[::step].sum())}:" \
                         f"{tensor_digest(torch.cat([pidx[:4096], pidx[-4096:]]))}"
-        del lors, y, pidx
+        del lors, y
         self.plans = [self.make_plan(self.subset(k)[0]) for k in range(self.n_sub)] if self.tiled else None
         self.done = 0   # sub-iterations run so far (over every run call)
 
@@ -314,6 +351,7 @@ class OsemEngine(EmEngine):
         events (permutation, count, sum of values), grid, TOF, EM constants,
         plan options and subset order."""
         return {"kind": "osem", "n_sub": self.n_sub, "n_per_sub": self.n_per_sub, "perm": self.perm_tag,
+                "event_order": self.event_order,
                 "values_sum": float(self.y.double().sum()), "shape": list(self.shape), "geom": list(self.geom),
                 "tof": [bool(self.use_tof), self.tof_bin, self.tof_sigma2], "ratio_kind": int(self.ratio_kind),
                 "eps": self.eps, "zero_guard": self.zero_guard, "tiled": self.tiled,
@@ -541,5 +579,5 @@ def crystal_pair_emap(scanner, image_config, chunk_pairs=1 << 24, item_records=8
 
 __all__ = ["Project", "ProjectTof", "BackProject", "BackProjectTof", "Mlem", "MlemFull", "MlemFullTof",
            "OsemFull", "OsemTof", "EmapGenerator", "CylinderScanner", "EmEngine", "OsemEngine",
-           "uniform_cylinder_lors", "subset_permutation", "to_gpu", "project_data", "backproject_data",
-           "crystal_pair_emap"]
+           "uniform_cylinder_lors", "subset_permutation", "geometric_subset_order", "to_gpu", "project_data",
+           "backproject_data", "crystal_pair_emap"]

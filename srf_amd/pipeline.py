@@ -229,7 +229,7 @@ def reconstruct_plain(scans, n_sub=8, n_iter=4, estimator=None, scatter="sss", i
         eng = OsemEngine(sc.lors, sc.meta["inv_att"], sc.emap, sc.shape, sc.geom, n_sub, perm, tof=tof,
                          ratio_kind=0, eps=1e-8, zero_guard=ZERO_GUARD, item_records=item_records,
                          tof_window=TOF_WINDOW)
-        pidx = perm[: eng.n_per_sub * n_sub]
+        pidx = eng.event_index
         for it in range(n_iter):
             if it >= 1 and scatter is not None:
                 torch.cuda.synchronize()
