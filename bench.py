@@ -65,6 +65,10 @@ def parse():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the MLEM (64x64x32, 1e6 LORs) and differentiable-projector (8 x 128x128x64, "
                          "4e6 LORs) GPU / CPU timings")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the live HBM traffic measurement (two rocprofv3 counter passes of one step)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="(run under rocprofv3 by the traffic measurement) a 1 GiB copy, then one OSEM step")
     ap.add_argument("--tof-window", type=float, default=22.0,
                     help="plan TOF window (terms below 2^-w of the event's TOF peak are not binned; 0 = all terms)")
     return ap.parse_args()
@@ -82,11 +86,82 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
+TRAFFIC_COPY = 1 << 30
+TRAFFIC_KERNELS = {"project": "k_fp_tiled", "backproject": "k_bp_tiled"}
+
+
+def _counter_means(csv_path, counter):
+    """(mean per-dispatch value of `counter` for each kernel name, the value
+    of the first device-copy dispatch) from a rocprofv3 counter CSV."""
+    import csv
+    acc, first_copy = {}, None
+    with open(csv_path) as f:
+        rows = [r for r in csv.DictReader(f) if r.get("Counter_Name") == counter]
+    rows.sort(key=lambda r: int(r.get("Dispatch_Id") or 0))
+    for r in rows:
+        name = r["Kernel_Name"]
+        acc.setdefault(name, []).append(float(r["Counter_Value"]))
+        if first_copy is None and "copyBuffer" in name:
+            first_copy = float(r["Counter_Value"])
+    return {k: sum(v) / len(v) for k, v in acc.items()}, first_copy
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of the tiled kernels, measured now: two child runs
+    of this script (a 1 GiB device copy, then one OSEM step of the same
+    workload) under rocprofv3, one with FETCH_SIZE, one with WRITE_SIZE.  The
+    copy calibrates each counter (bytes copied / counter bytes).  None when
+    rocprofv3 is absent or a pass fails."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    child = [sys.executable, os.path.join(ROOT, "bench.py"), "--traffic-probe", "--events", str(args.events),
+             "--subsets", str(args.subsets), "--seed", str(args.seed), "--tof-window", str(args.tof_window)]
+    res = {}
+    with tempfile.TemporaryDirectory() as td:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(td, counter)
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--"] + child
+            log(f"traffic: {counter} pass")
+            try:
+                subprocess.run(cmd, check=True, timeout=300, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                               env=dict(os.environ, TMPDIR=td))
+            except (subprocess.SubprocessError, OSError) as e:
+                log(f"traffic: {counter} pass failed ({e})")
+                return None
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            if not files:
+                return None
+            means, copy = _counter_means(files[0], counter)
+            if not copy:
+                return None
+            # the probe's copy (the process's first) moves TRAFFIC_COPY bytes each way
+            factor = TRAFFIC_COPY / (copy * 1024.0)   # bytes moved per counted byte
+            for part, kname in TRAFFIC_KERNELS.items():
+                vals = [v for k, v in means.items() if kname in k]
+                if vals:
+                    res.setdefault(part, {})[counter] = vals[0] * 1024.0 * factor
+            res.setdefault("factor", {})[counter] = factor
+    out = {part: d["FETCH_SIZE"] + d["WRITE_SIZE"] for part, d in res.items()
+           if part != "factor" and "FETCH_SIZE" in d and "WRITE_SIZE" in d}
+    out["parts"] = {k: v for k, v in res.items() if k != "factor"}
+    out["calibration_factor"] = res.get("factor")
+    return out
+
+
 def main():
     args = parse()
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.traffic_probe:
+        copy_probe(TRAFFIC_COPY)
+        args.steps, args.warmup = 1, 0
+        args.no_cpu = args.no_scatter = args.no_batch = args.no_emap = args.no_configs = args.no_traffic = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs (events sharded "
               "across GPUs, one plan per share) are not implemented", file=sys.stderr)
@@ -227,15 +302,8 @@ def main():
     fp_bw = fp_bytes / (fp_ms * 1e-3)
     bp_bw = bp_bytes / (bp_ms * 1e-3)
     dom = ("backproject", bp_ms, bp_bw, bp_bytes) if bp_ms >= fp_ms else ("project", fp_ms, fp_bw, fp_bytes)
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_tof_osem_256x256x128.json")
-    if os.path.exists(tf):
-        try:
-            traffic = json.load(open(tf)).get(dom[0])
-        except Exception:
-            traffic = None
     roof = {"bound": "hbm", "achieved": dom[2] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": dom[2] / HBM_PEAK, "traffic": traffic, "kernel": dom[0], "mean_launch_ms": dom[1],
+            "frac": dom[2] / HBM_PEAK, "traffic": None, "kernel": dom[0], "mean_launch_ms": dom[1],
             "algorithmic_bytes_per_launch": dom[3],
             "other": {"project_ms": fp_ms, "backproject_ms": bp_ms,
                       # whole halves: + per-event reduction / + tile sums and EM update
@@ -256,6 +324,23 @@ def main():
         p.close()
     del eng
     torch.cuda.empty_cache()
+    if args.traffic_probe:
+        return
+    # traffic: measured now by two counter passes of one step; the committed
+    # profile's figure only when no pass can run
+    tr = None if (args.no_traffic or args.event_kernels) else measure_traffic(args)
+    if tr is not None and dom[0] in tr:
+        roof["traffic"] = tr[dom[0]]
+        roof["traffic_source"] = "rocprofv3 FETCH_SIZE + WRITE_SIZE passes of this run, copy-calibrated"
+        roof["traffic_parts"] = tr
+    else:
+        tf = os.path.join(ROOT, "profiles", "traffic_tof_osem_256x256x128.json")
+        if os.path.exists(tf) and not args.event_kernels:
+            try:
+                roof["traffic"] = json.load(open(tf)).get(dom[0])
+                roof["traffic_source"] = "profiles/traffic_tof_osem_256x256x128.json (no live counter pass)"
+            except (OSError, ValueError):
+                pass
     scatter = None if args.no_scatter else scatter_bench(args)
     batch = None if args.no_batch else batch_bench(args)
     emap = None if args.no_emap else emap_bench(args)
