@@ -497,12 +497,16 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         } else {
             const bool ts1 = (unsigned)(cs1 - s0) < es, ts2 = (unsigned)(cs2 - s0) < es;
             const bool tz1 = (unsigned)(cz1 - z0) < ez, tz2 = (unsigned)(cz2 - z0) < ez;
-            const int A = (int)(__umul24((unsigned)cs1, BS) + (unsigned)base);
-            const int B = (int)(__umul24((unsigned)cs2, BS) + (unsigned)base);
-            i1 = (ts1 && tz1) ? (int)(__umul24((unsigned)cz1, BZ) + (unsigned)A) : dummy;
-            i3 = ((bs || bz) && ts2 && tz2 && los && loz) ? (int)(__umul24((unsigned)cz2, BZ) + (unsigned)B) : dummy;
-            i2a = (bs && bz && ts1 && tz2 && loz) ? (int)(__umul24((unsigned)cz2, BZ) + (unsigned)A) : dummy;
-            i2b = (bs && bz && ts2 && tz1 && los) ? (int)(__umul24((unsigned)cz1, BZ) + (unsigned)B) : dummy;
+            // six signed 24-bit multiply-adds, all computed before the
+            // selects (an address is used only when its cells pass the tile
+            // tests, so lie in [origin, origin + extent))
+            const int A = mad_i24(cs1, bs_v, base), B = mad_i24(cs2, bs_v, base);
+            const int j1 = mad_i24(cz1, bz_v, A), j3 = mad_i24(cz2, bz_v, B);
+            const int j2a = mad_i24(cz2, bz_v, A), j2b = mad_i24(cz1, bz_v, B);
+            i1 = (ts1 && tz1) ? j1 : dummy;
+            i3 = ((bs || bz) && ts2 && tz2 && los && loz) ? j3 : dummy;
+            i2a = (bs && bz && ts1 && tz2 && loz) ? j2a : dummy;
+            i2b = (bs && bz && ts2 && tz1 && los) ? j2b : dummy;
         }
         // materialise the addresses here: sunk past the branch below, their
         // lane-mask operands would be carried across it as 0/1 vectors
