@@ -5,7 +5,8 @@
  *  - All pointers are device pointers (HBM) unless the comment says otherwise.
  *    The caller owns every buffer; the library allocates device memory only in
  *    srf_plan_create[_ex], srf_plan_rebuild and srf_batch_create (freed by the
- *    matching *_destroy) and transient scratch in srf_sinogram_lors.
+ *    matching *_destroy) and transient scratch in srf_sinogram_lors and
+ *    srf_event_order.
  *  - LORs are AoS rows of `lor_stride` floats [x1,y1,z1,x2,y2,z2,(tof,...)] in
  *    mm, exactly Lors.data of srfnef/data/lors.py:12-22 (stride 6 or 7).
  *  - TOF values are read from `tof_values[e * tof_stride]`; pass
@@ -224,6 +225,23 @@ int srf_plan_destroy(srf_plan *plan);
  * memory no live plan holds.  No reference counterpart (the plan pool is this
  * library's). */
 int srf_plan_pool_trim(size_t keep_bytes);
+/* Geometric event order within OSEM subsets.  pidx holds n event indices
+ * into the LOR array (n_lors rows of lor_stride floats) in n_sub contiguous
+ * subsets of n / n_sub entries (a remainder forms one more group, last).
+ * out = pidx with each subset's entries stably sorted by the geometry of
+ * their LORs: transaxial angle of the direction folded into [0, pi) (128
+ * bins), axial midpoint (16 mm bins), signed transaxial distance from the
+ * axis (1 mm bins).  Subset membership is unchanged, and with it every EM
+ * update; events that cross the same image tile get nearby positions, so a
+ * plan built from the ordered subset gathers and scatters its per-event
+ * values over few cache lines.  The reference keeps its subsets in the order
+ * of the shuffled permutation (srfnef/functions/osem_full.py:63-73): this
+ * reorders within a subset only.  n < 2^31; out must not alias pidx.  Scratch
+ * (about 32 B per entry) comes from the plan pool; the call synchronises
+ * `stream`.  An index outside [0, n_lors) returns SRF_ERR_INVALID_ARG (out
+ * is then unspecified). */
+int srf_event_order(const float *lors, int64_t n_lors, int64_t lor_stride, const int64_t *pidx, int64_t n,
+                    int64_t n_sub, int64_t *out, void *stream);
 int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments,
                   int64_t *n_items, size_t *device_bytes);
 /* out[e] = projection of event e (srf_project semantics). */

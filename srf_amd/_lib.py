@@ -100,6 +100,7 @@ SIGNATURES = [
                                          _P]),
     ("srf_plan_destroy", ctypes.c_int, [_P]),
     ("srf_plan_pool_trim", ctypes.c_int, [ctypes.c_size_t]),
+    ("srf_event_order", ctypes.c_int, [_P, _I64, _I64, _P, _I64, _I64, _P, _P]),
     ("srf_batch_create", ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(SrfBatchScan), _I32,
                                         ctypes.POINTER(SrfBatchCfg), ctypes.POINTER(_P), _P]),
     ("srf_batch_destroy", ctypes.c_int, [_P]),

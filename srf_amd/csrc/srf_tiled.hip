@@ -1108,6 +1108,50 @@ __global__ void k_iota_i32(int *__restrict__ d, int64_t n)
     if (i < n) d[i] = (int)i;
 }
 
+// Geometric event order (srf_event_order).  One 64-bit key per entry j of the
+// caller's index list: bits 33.. the subset j / nps (clamped to n_sub, so a
+// remainder sorts last), then the transaxial angle of the LOR direction folded
+// into [0, pi) (7 bits, 128 bins), the axial midpoint (10 bits, 16 mm bins,
+// offset 512) and the signed transaxial distance from the axis, negated with
+// the folded direction (16 bits, 1 mm bins, offset 32768).  Bins saturate at
+// their range; a NaN coordinate gives bin 0.  An index outside [0, n_lors)
+// raises *bad and keys only its subset.
+__device__ __forceinline__ unsigned order_bin(float v, float hi)
+{
+    return v != v ? 0u : (unsigned)fminf(fmaxf(v, 0.0f), hi);
+}
+
+__global__ void k_event_keys(const float *__restrict__ lors, int64_t n_lors, int64_t lor_stride,
+                             const int64_t *__restrict__ pidx, int64_t n, int64_t nps, int64_t n_sub,
+                             uint64_t *__restrict__ key, int *__restrict__ bad)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int64_t e = pidx[j];
+    const int64_t sub = j / nps < n_sub ? j / nps : n_sub;
+    uint64_t k = (uint64_t)sub << 33;
+    if (e < 0 || e >= n_lors) {
+        *bad = 1;
+        key[j] = k;
+        return;
+    }
+    const float *L = lors + e * lor_stride;
+    const float x1 = L[0], y1 = L[1], z1 = L[2];
+    const float dx = L[3] - x1, dy = L[4] - y1;
+    float phi = atan2f(dy, dx);
+    const bool flip = phi < 0.0f;
+    if (flip) phi += 3.14159265358979f;
+    const float h = sqrtf(dx * dx + dy * dy);
+    float r = (x1 * dy - y1 * dx) / (h > 0.0f ? h : 1.0f);
+    if (flip) r = -r;
+    const float zm = 0.5f * (z1 + L[5]);
+    const unsigned ab = order_bin(floorf(phi * (float)(128.0 / 3.14159265358979323846)), 127.0f);
+    const unsigned zb = order_bin(floorf(zm * 0.0625f) + 512.0f, 1023.0f);
+    const unsigned rb = order_bin(floorf(r) + 32768.0f, 65535.0f);
+    k |= ((uint64_t)ab << 26) | ((uint64_t)zb << 16) | rb;
+    key[j] = k;
+}
+
 // Records permuted into the order `idx`; their event ids (the records' own
 // ev field) are written beside them for the backprojector's value gathers.
 __global__ void k_permute_recs(const SegRec *__restrict__ recs, const int *__restrict__ idx, int64_t n,
@@ -1715,6 +1759,44 @@ int srf_plan_pool_trim(size_t keep_bytes)
         return SRF_ERR_LAUNCH;
     }
     return SRF_OK;
+}
+
+int srf_event_order(const float *lors, int64_t n_lors, int64_t lor_stride, const int64_t *pidx, int64_t n,
+                    int64_t n_sub, int64_t *out, void *stream)
+{
+    if (n < 0 || n_lors < 0 || n_sub < 1 || lor_stride < 6) return SRF_ERR_INVALID_ARG;
+    if (n == 0) return SRF_OK;
+    if (!lors || !pidx || !out || out == pidx) return SRF_ERR_INVALID_ARG;
+    if (n >= INT32_MAX || n_sub >= ((int64_t)1 << 30)) return SRF_ERR_UNSUPPORTED;
+    const hipStream_t s = (hipStream_t)stream;
+    const int64_t nps = n / n_sub > 0 ? n / n_sub : 1;
+    int sub_bits = 1;
+    while (((int64_t)1 << sub_bits) <= n_sub) ++sub_bits;
+    DevBuf<uint64_t> key, key2;
+    DevBuf<unsigned char> tmp;
+    DevBuf<int> bad;
+    auto cleanup = [&]() {
+        dfree(key, s);
+        dfree(key2, s);
+        dfree(tmp, s);
+        dfree(bad, s);
+    };
+    SRF_TRY(ensure(key, (size_t)n, s));
+    SRF_TRY(ensure(key2, (size_t)n, s));
+    SRF_TRY(ensure(bad, 1, s));
+    SRF_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_event_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lors, n_lors, lor_stride,
+                       pidx, n, nps, n_sub, key.p, bad.p);
+    SRF_HIP(hipGetLastError());
+    size_t bytes = 0;
+    SRF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, key2.p, pidx, out, (int)n, 0, 33 + sub_bits, s));
+    SRF_TRY(ensure(tmp, bytes, s));
+    SRF_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, key2.p, pidx, out, (int)n, 0, 33 + sub_bits, s));
+    int h_bad = 0;
+    SRF_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SRF_HIP(hipStreamSynchronize(s));
+    cleanup();
+    return h_bad ? SRF_ERR_INVALID_ARG : SRF_OK;
 }
 
 int srf_plan_info(const srf_plan *plan, int64_t *n_events, int64_t *n_segments, int64_t *n_items,

@@ -275,31 +275,24 @@ def geometric_subset_order(lors, pidx, n_sub):
     """The subsets of ``pidx`` (n_sub contiguous chunks of event indices) with
     each subset's events ordered by LOR geometry: transaxial angle (128 bins
     over [0, pi)), axial midpoint (16 mm bins), then signed transaxial
-    distance from the axis (1 mm bins), a stable sort.  Membership, and with
-    it every EM update, is unchanged (the plans' sums do not depend on event
-    order); events that cross the same image tile get nearby indices, so the
-    tiled kernels' per-event reads and writes (ratios, partial sums) touch a
-    few cache lines per tile instead of one per event."""
-    n = int(pidx.numel())
+    distance from the axis (1 mm bins), a stable sort (srf_event_order).
+    Membership, and with it every EM update, is unchanged (the plans' sums do
+    not depend on event order); events that cross the same image tile get
+    nearby indices, so the tiled kernels' per-event reads and writes (ratios,
+    partial sums) touch a few cache lines per tile instead of one per event."""
+    from ._lib import check, lib, stream_handle
+    idx = pidx.to(torch.long).contiguous()
+    n = int(idx.numel())
     if n == 0:
-        return pidx
-    nps = n // int(n_sub)
-    L = lors.index_select(0, pidx)
-    p1, d = L[:, 0:3], L[:, 3:6] - L[:, 0:3]
-    phi = torch.atan2(d[:, 1], d[:, 0])
-    flip = phi < 0
-    phi = torch.where(flip, phi + math.pi, phi)
-    h = torch.hypot(d[:, 0], d[:, 1])
-    r = (p1[:, 0] * d[:, 1] - p1[:, 1] * d[:, 0]) / torch.where(h > 0, h, torch.ones_like(h))
-    r = torch.where(flip, -r, r)
-    zm = 0.5 * (L[:, 2] + L[:, 5])
-    ab = torch.nan_to_num((phi * (128.0 / math.pi)).floor(), 0.0).clamp(0, 127).long()
-    zb = torch.nan_to_num((zm / 16.0).floor() + 512, 0.0).clamp(0, 1023).long()
-    rb = torch.nan_to_num(r.floor() + 32768, 0.0).clamp(0, 65535).long()
-    sub = torch.arange(n, device=pidx.device) // max(nps, 1)
-    key = ((sub * 128 + ab) * 1024 + zb) * 65536 + rb
-    order = torch.sort(key, stable=True).indices
-    return pidx.index_select(0, order)
+        return idx
+    L = lors.contiguous()
+    if not L.is_cuda or L.dtype != torch.float32 or L.dim() != 2 or idx.device != L.device:
+        raise ValueError("geometric_subset_order: float32 (N, 6|7) LORs and indices on one GPU")
+    out = torch.empty_like(idx)
+    with torch.cuda.device(L.device):
+        check(lib().srf_event_order(L.data_ptr(), int(L.shape[0]), int(L.shape[1]), idx.data_ptr(), n, int(n_sub),
+                                    out.data_ptr(), stream_handle(L.device)), "srf_event_order")
+    return out
 
 
 def subset_permutation(n, seed):
@@ -334,10 +327,7 @@ class OsemEngine(EmEngine):
         super().__init__(lors.index_select(0, pidx).contiguous(), y.index_select(0, pidx).contiguous(),
                          emap_data, shape, geom, tof=tof, ratio_kind=ratio_kind, eps=eps, zero_guard=zero_guard,
                          tiled=tiled, plan=False, item_records=item_records, tof_window=tof_window)
-        # the permutation, named by a cheap fingerprint (checkpoint keys)
-        step = max(1, pidx.numel() // 65536)
-        self.perm_tag = f"{int(pidx.numel())}:{int(pidx[::step].sum())}:" \
-                        f"{tensor_digest(torch.cat([pidx[:4096], pidx[-4096:]]))}"
+        self._perm_tag = None
         del lors, y
         self.plans = [self.make_plan(self.subset(k)[0]) for k in range(self.n_sub)] if self.tiled else None
         self.done = 0   # sub-iterations run so far (over every run call)
@@ -345,6 +335,17 @@ class OsemEngine(EmEngine):
     def subset(self, k):
         a, b = k * self.n_per_sub, (k + 1) * self.n_per_sub
         return self.lors[a:b], self.y[a:b]
+
+    @property
+    def perm_tag(self):
+        """The event order, named by a cheap fingerprint (checkpoint keys);
+        computed on first use, so that construction does not wait for it."""
+        if self._perm_tag is None:
+            p = self.event_index
+            step = max(1, p.numel() // 65536)
+            self._perm_tag = f"{int(p.numel())}:{int(p[::step].sum())}:" \
+                             f"{tensor_digest(torch.cat([p[:4096], p[-4096:]]))}"
+        return self._perm_tag
 
     def checkpoint_key(self, reference_subset_order):
         """What a checkpoint of this engine must match to be resumed: the

@@ -61,6 +61,11 @@ def test_argument_validation_returns_errors(L):
     assert L.srf_em_update(ctypes.c_void_p(18), fake, fake, 4, 0.0, None) == -1   # misaligned
     assert L.srf_lor_digest(fake, 5, 6, None, 0, g, None, 3, fake, fake, None) == -1
     assert L.srf_image_to_kernel_layout(None, fake, 4, 4, 4, None) == -1
+    # event order: no subsets, short stride, output aliasing the input, too many entries
+    assert L.srf_event_order(fake, 5, 6, fake, 5, 0, ctypes.c_void_p(32), None) == -1
+    assert L.srf_event_order(fake, 5, 5, fake, 5, 1, ctypes.c_void_p(32), None) == -1
+    assert L.srf_event_order(fake, 5, 6, fake, 5, 1, fake, None) == -1
+    assert L.srf_event_order(fake, 5, 6, fake, 1 << 31, 1, ctypes.c_void_p(32), None) == -2
 
 
 def test_empty_inputs_are_no_ops(L):
@@ -70,3 +75,4 @@ def test_empty_inputs_are_no_ops(L):
     assert L.srf_ratio(None, None, None, 0, 0, 0.0, None) == 0
     assert L.srf_em_update(None, None, None, 0, 0.0, None) == 0
     assert L.srf_lor_digest(None, 0, 6, None, 0, g, None, 0, None, None, None) == 0
+    assert L.srf_event_order(None, 0, 6, None, 0, 4, None, None) == 0

@@ -107,3 +107,71 @@ def test_osem_event_order_within_subsets():
     for k in range(4):
         assert np.array_equal(np.sort(a[k * nps:(k + 1) * nps]), np.sort(b[k * nps:(k + 1) * nps]))
     assert np.array_equal(engs[0].y.cpu().numpy(), y.cpu().numpy()[a])
+
+
+def _order_keys(L, pos, nps, n_sub):
+    """float64 restatement of the geometric order's key for events with LORs L
+    at positions pos of the index list, and whether any of its three values
+    lies within 1e-3 bin of a bin edge (where float32 may round either way)."""
+    L = L.astype(np.float64)
+    dx, dy = L[:, 3] - L[:, 0], L[:, 4] - L[:, 1]
+    with np.errstate(invalid="ignore"):
+        phi = np.arctan2(dy, dx)
+        flip = phi < 0
+        phi = np.where(flip, phi + np.pi, phi)
+        h = np.hypot(dx, dy)
+        r = (L[:, 0] * dy - L[:, 1] * dx) / np.where(h > 0, h, 1.0)
+        r = np.where(flip, -r, r)
+        zm = 0.5 * (L[:, 2] + L[:, 5])
+        near = np.zeros(len(L), bool)
+        bins = []
+        for v, hi in ((phi * 128 / np.pi, 127), (zm / 16 + 512, 1023), (r + 32768, 65535)):
+            near |= np.abs(v - np.round(v)) < 1e-3
+            bins.append(np.clip(np.nan_to_num(np.floor(v), nan=0.0), 0, hi).astype(np.int64))
+    sub = np.minimum(pos // nps, n_sub)
+    return ((sub * 128 + bins[0]) * 1024 + bins[1]) * 65536 + bins[2], near
+
+
+def test_event_order_sorts_each_subset_stably():
+    """srf_event_order on 250003 indices into 3e5 LORs (1000 of them NaN), 4
+    subsets and a remainder of 3: every subset keeps its members; within it
+    the keys recomputed in float64 ascend, NaN LORs included, and equal keys
+    keep the caller's order (a stable sort), except next to a bin edge where
+    float32 may round the other way; the remainder stays last; an index
+    outside the LOR array is refused."""
+    from srf_amd._lib import SrfError
+    from srf_amd.functions import geometric_subset_order
+    rng = np.random.default_rng(11)
+    n_lors, n, n_sub = 300_000, 250_003, 4
+    a = rng.uniform(0, 2 * np.pi, (n_lors, 2))
+    z = rng.uniform(-150, 150, (n_lors, 2))
+    L = np.stack([300 * np.cos(a[:, 0]), 300 * np.sin(a[:, 0]), z[:, 0],
+                  300 * np.cos(a[:, 1]), 300 * np.sin(a[:, 1]), z[:, 1], rng.normal(0, 50, n_lors)], 1)
+    L = L.astype(np.float32)
+    nan_rows = rng.choice(n_lors, 1000, replace=False)
+    L[nan_rows, rng.integers(0, 6, 1000)] = np.nan
+    perm = rng.permutation(n_lors)[:n]
+    Lg = torch.from_numpy(L).cuda()
+    out = geometric_subset_order(Lg, torch.from_numpy(perm).cuda(), n_sub).cpu().numpy()
+    assert np.array_equal(out, geometric_subset_order(Lg, torch.from_numpy(perm).cuda(), n_sub).cpu().numpy())
+    nps = n // n_sub
+    groups = [(k * nps, (k + 1) * nps) for k in range(n_sub)] + [(n_sub * nps, n)]
+    for s, e in groups:
+        assert np.array_equal(np.sort(out[s:e]), np.sort(perm[s:e]))
+    where = np.empty(n_lors, np.int64)
+    where[perm] = np.arange(n)                       # position of each event in the caller's list
+    key, near = _order_keys(L[out], where[out], nps, n_sub)
+    same_group = np.minimum(where[out][1:] // nps, n_sub) == np.minimum(where[out][:-1] // nps, n_sub)
+    exact = same_group & ~near[1:] & ~near[:-1]
+    assert exact.mean() > 0.97       # each of 3 values is near an edge with probability 2e-3
+    assert bool((key[1:] >= key[:-1])[exact].all())
+    ties = exact & (key[1:] == key[:-1])
+    assert ties.sum() > 0 and bool((where[out][1:] > where[out][:-1])[ties].all())
+    isnan = np.isnan(L[out]).any(1)                 # their NaN values fall into bin 0
+    assert isnan.sum() > 700 and (exact & (isnan[1:] | isnan[:-1])).sum() > 0.9 * isnan.sum()
+    print(f"event order: {int(exact.sum())} of {n - 1} neighbour pairs away from bin edges, {int(ties.sum())} ties",
+          flush=True)
+    bad = torch.from_numpy(perm.copy()).cuda()
+    bad[7] = n_lors
+    with pytest.raises(SrfError, match="invalid"):
+        geometric_subset_order(Lg, bad, n_sub)

@@ -63,32 +63,6 @@ def test_recon_from_json(tmp_path):
     assert type(recon.recon_from_dict(cfg)).__name__ == "MlemFullTof"
 
 
-def test_geometric_subset_order_keeps_subsets():
-    """Each subset keeps its events; within it they are ordered by (angle,
-    axial midpoint, transaxial distance) bins; the order is reproducible."""
-    import math
-    import torch
-    from srf_amd.functions import geometric_subset_order
-    g = torch.Generator().manual_seed(0)
-    n, n_sub = 6000, 4
-    a1, a2 = torch.rand(n, generator=g) * 2 * math.pi, torch.rand(n, generator=g) * 2 * math.pi
-    z1, z2 = torch.rand(n, generator=g) * 200 - 100, torch.rand(n, generator=g) * 200 - 100
-    lors = torch.stack([300 * torch.cos(a1), 300 * torch.sin(a1), z1, 300 * torch.cos(a2), 300 * torch.sin(a2), z2],
-                       1)
-    perm = torch.randperm(n, generator=g)[: (n // n_sub) * n_sub]
-    out = geometric_subset_order(lors, perm, n_sub)
-    nps = n // n_sub
-    for k in range(n_sub):
-        assert sorted(out[k * nps:(k + 1) * nps].tolist()) == sorted(perm[k * nps:(k + 1) * nps].tolist())
-        L = lors[out[k * nps:(k + 1) * nps]]
-        d = L[:, 3:6] - L[:, 0:3]
-        phi = torch.atan2(d[:, 1], d[:, 0])
-        phi = torch.where(phi < 0, phi + math.pi, phi)
-        ab = (phi * (128.0 / math.pi)).floor().clamp(0, 127)
-        assert bool((ab[1:] >= ab[:-1]).all())          # angle bins ascend within a subset
-    assert torch.equal(out, geometric_subset_order(lors, perm, n_sub))
-
-
 DOC_V7 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "doc_v7_recon.json")
 
 
