@@ -98,9 +98,12 @@ WEIGHTS_IN_LOOP = os.path.join(ROOT, "srf_amd", "weights", "scatter_unet_in_loop
 
 def test_in_loop_model_on_held_out_osem_iterates():
     """The hybrid in-loop model (coarse-SSS fifth channel) against SSS on OSEM
-    iterates of held-out simulated scans (seeds disjoint from training): the
-    median within the 2% check and at least 60% of the pairs within it (the
-    training run measured 1.15% median and 87% on 150 such pairs)."""
+    iterates of held-out simulated scans (seeds disjoint from training): on
+    these 18 seeded pairs the median below 1.5%, at least 90% within the 2%
+    check and none above 3% (measured: 1.07% median, 1.27% max, all within
+    2%; the training run measured 1.15% median and 87% within 2% on 150
+    pairs, and the in-loop checks of the 32-scan workload 1.0% median, 3.2%
+    max)."""
     from srf_amd import train_surrogate as ts
     from srf_amd.surrogate import UNet, relative_l2
     from srf_amd.sss import SssGeometry
@@ -113,8 +116,9 @@ def test_in_loop_model_on_held_out_osem_iterates():
     errs = np.asarray([relative_l2(net.forward(X[i], dims) * (M[i] / meta["kappa"]), S[i]) for i in range(18)])
     print(f"in-loop held-out: mean {errs.mean():.4f} median {np.median(errs):.4f} max {errs.max():.4f} "
           f"<=2%: {(errs <= 0.02).mean():.2f}", flush=True)
-    assert np.median(errs) <= 0.02
-    assert (errs <= 0.02).mean() >= 0.6
+    assert np.median(errs) <= 0.015
+    assert (errs <= 0.02).mean() >= 0.9
+    assert errs.max() <= 0.03
 
 
 def test_ring_subset_sss_matches_full(setup):
