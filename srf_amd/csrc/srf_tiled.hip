@@ -76,16 +76,17 @@ struct TileCfg {
     int ntx, nty, ntz, ntiles, nkeys;
 };
 
+// One segment: 48 bytes, read as three 16-byte loads.  Its event id is kept
+// beside the records (rec_ev), where only the backprojector's value gathers
+// and the digest read it.
 struct __align__(16) SegRec {
     float p1, sec1, ks, z1;
     float kz, Wbp, Wfp;
     float tA, tB;            // TOF: exp(-d^2 / (2 tse)) == exp2(-(pp1 * tA + tB)^2), pp1 = ip - np2
-    int ev, slot;
-    uint32_t ab_bp, ab_fp;   // slab range a | b << 16 (backprojector / projector)
-    uint32_t meta;           // bit 0: x-dominant branch
-    uint32_t pad[2];
+    int slot;
+    uint32_t ab_bp, ab_fp;   // slab range a | x-dominant branch << 15 | b << 16 (backprojector / projector)
 };
-static_assert(sizeof(SegRec) == 64, "record is one 64-byte line");
+static_assert(sizeof(SegRec) == 48, "record is three 16-byte words");
 
 struct Item {
     int start, count, tile, cls;   // cls: slab-count class of the item's key
@@ -218,7 +219,14 @@ __device__ __forceinline__ void for_each_event_segment(const RaySetup &r, const 
     }
 }
 
-__device__ __forceinline__ uint32_t pack_ab(int a, int b) { return (uint32_t)a | ((uint32_t)b << 16); }
+// Slab range [a, b) of a record and its branch (slab indices below 2^15).
+__device__ __forceinline__ uint32_t pack_ab(int a, int b, bool xd)
+{
+    return (uint32_t)a | ((uint32_t)xd << 15) | ((uint32_t)b << 16);
+}
+__device__ __forceinline__ int ab_a(uint32_t ab) { return (int)(ab & 0x7fffu); }
+__device__ __forceinline__ int ab_b(uint32_t ab) { return (int)(ab >> 16); }
+__device__ __forceinline__ bool ab_xd(uint32_t ab) { return (ab >> 15) & 1u; }
 
 // TOF Gaussian of siddon_tof.cu.cc:55-59 as a linear function of the slab's
 // lower boundary pp1 = ip - np2 (exact in float):
@@ -285,7 +293,8 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
                                                           const float *__restrict__ tofv, int tof_stride,
                                                           LaunchGeom G, TileCfg T, float twin, int prescale,
                                                           const int *__restrict__ ev_off,
-                                                          SegRec *__restrict__ recs, unsigned *__restrict__ rec_sk)
+                                                          SegRec *__restrict__ recs, int *__restrict__ slot_ev,
+                                                          unsigned *__restrict__ rec_sk)
 {
     const int64_t e = (int64_t)blockIdx.x * BIN_THREADS + threadIdx.x;
     if (e >= n) return;
@@ -309,19 +318,17 @@ __global__ __launch_bounds__(BIN_THREADS) void k_bin_emit(const float *__restric
         R.Wbp = branch_weight<TOF>(B, r, G, false);
         R.Wfp = branch_weight<TOF>(B, r, G, true);
         tof_coeffs<TOF>(B, r, G, R.tA, R.tB);
-        R.ev = (int)e;
         R.slot = pos;
-        R.ab_bp = c.bp_use ? pack_ab(a, b) : 0u;
+        R.ab_bp = c.bp_use ? pack_ab(a, b, B.xdom) : 0u;
         const int fa = max(a, c.fa), fb = min(b, c.fb);
-        R.ab_fp = (c.fp_use && fa < fb) ? pack_ab(fa, fb) : 0u;
-        R.meta = B.xdom ? 1u : 0u;
-        R.pad[0] = R.pad[1] = 0u;
+        R.ab_fp = (c.fp_use && fa < fb) ? pack_ab(fa, fb, B.xdom) : 0u;
         recs[pos] = R;
+        slot_ev[pos] = (int)e;
         // sort key: (tile, class), then the branch (a wavefront mostly runs
         // one branch's marcher), then slab count (longest first: its lanes
         // march similar lengths); a stable sort keeps slot order within a key
         const unsigned len = (unsigned)min(b - a, 63);
-        rec_sk[pos] = ((unsigned)(tile * NCLASS + cls) << 7) | (R.meta << 6) | (63u - len);
+        rec_sk[pos] = ((unsigned)(tile * NCLASS + cls) << 7) | ((unsigned)B.xdom << 6) | (63u - len);
     });
 }
 
@@ -611,10 +618,10 @@ __device__ __forceinline__ void tile_origin(int tile, const TileCfg &T, int &x0,
 // Both branches' marchers; a record's branch bit picks one (the records of a
 // work item are sorted by branch, so a wavefront mostly runs one of them).
 template <bool TOF, bool DS, bool DZ, int MODE>
-__device__ __forceinline__ void march_any(const SegRec &R, const LaunchGeom &G, int a, int b, float W, TileWalk &t)
+__device__ __forceinline__ void march_any(const SegRec &R, const LaunchGeom &G, uint32_t ab, float W, TileWalk &t)
 {
-    if (R.meta & 1u) march_tile<TOF, DS, DZ, MODE, true>(R, G, a, b, W, t);
-    else march_tile<TOF, DS, DZ, MODE, false>(R, G, a, b, W, t);
+    if (ab_xd(ab)) march_tile<TOF, DS, DZ, MODE, true>(R, G, ab_a(ab), ab_b(ab), W, t);
+    else march_tile<TOF, DS, DZ, MODE, false>(R, G, ab_a(ab), ab_b(ab), W, t);
 }
 
 __device__ __forceinline__ void walk_for(TileWalk &t, float *lds, int dummy, int x0, int y0, int z0)
@@ -680,7 +687,7 @@ __device__ __forceinline__ void fp_item(const SegRec *__restrict__ recs, const I
         if (r + TILE_THREADS < end) Rn = recs[r + TILE_THREADS];
         TileWalk t;
         walk_for(t, tile, HPAD, x0, y0, z0);
-        march_any<TOF, DS, DZ, M_GATHER>(R, G, (int)(R.ab_fp & 0xffffu), (int)(R.ab_fp >> 16), R.Wfp, t);
+        march_any<TOF, DS, DZ, M_GATHER>(R, G, R.ab_fp, R.Wfp, t);
         partial[R.slot] = t.acc;
     }
 }
@@ -787,13 +794,12 @@ __device__ __forceinline__ void bp_item_bands(const SegRec *__restrict__ recs, c
             if (v == 0.0f || value_band(v, etop, nb) != band) continue;
             const SegRec R = recs[r];
             const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
-            const int a = (int)(ab & 0xffffu), b = (int)(ab >> 16);
-            if (a >= b) continue;
+            if (ab_a(ab) >= ab_b(ab)) continue;
             TileWalk t;
             walk_for(t, fl, TPAD, x0, y0, z0);
             t.fx = fx;
             t.value = ldexpf(v, shift);
-            march_any<TOF, DS, DZ, M_SCATTER_FX>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
+            march_any<TOF, DS, DZ, M_SCATTER_FX>(R, G, ab, adjoint ? R.Wfp : R.Wbp, t);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < TVOX; i += TILE_THREADS) {
@@ -873,14 +879,13 @@ __device__ __forceinline__ void bp_item(const SegRec *__restrict__ recs, const I
         const SegRec R = Rn;
         if (r + TILE_THREADS < end) Rn = recs[r + TILE_THREADS];
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
-        const int a = (int)(ab & 0xffffu), b = (int)(ab >> 16);
-        if (a >= b) continue;
+        if (ab_a(ab) >= ab_b(ab)) continue;
         TileWalk t;
         walk_for(t, fl, TPAD, x0, y0, z0);
         t.fx = fx;
         t.value = fixed ? ldexpf(v, shift) : v;   // scale folded into the value
-        if (fixed) march_any<TOF, DS, DZ, M_SCATTER_FX>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
-        else march_any<TOF, DS, DZ, M_SCATTER>(R, G, a, b, adjoint ? R.Wfp : R.Wbp, t);
+        if (fixed) march_any<TOF, DS, DZ, M_SCATTER_FX>(R, G, ab, adjoint ? R.Wfp : R.Wbp, t);
+        else march_any<TOF, DS, DZ, M_SCATTER>(R, G, ab, adjoint ? R.Wfp : R.Wbp, t);
     }
     __syncthreads();
     if (fixed) {
@@ -912,6 +917,7 @@ __global__ __launch_bounds__(TILE_THREADS) BP_OCCUPANCY void k_bp_tiled(const Se
 
 template <bool TOF, bool DS, bool DZ>
 __global__ __launch_bounds__(TILE_THREADS) void k_digest_tiled(const SegRec *__restrict__ recs,
+                                                               const int *__restrict__ rec_ev,
                                                                const Item *__restrict__ items, int adjoint,
                                                                LaunchGeom G, TileCfg T,
                                                                unsigned long long *__restrict__ hash,
@@ -925,10 +931,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_digest_tiled(const SegRec *__r
         const uint32_t ab = adjoint ? R.ab_fp : R.ab_bp;
         TileWalk t;
         walk_for(t, nullptr, TPAD, x0, y0, z0);   // dummy: never a tile address
-        march_any<TOF, DS, DZ, M_DIGEST>(R, G, (int)(ab & 0xffffu), (int)(ab >> 16), 0.0f, t);
+        march_any<TOF, DS, DZ, M_DIGEST>(R, G, ab, 0.0f, t);
         if (t.cnt) {
-            atomicAdd(hash + R.ev, (unsigned long long)t.h);
-            atomicAdd(count + R.ev, t.cnt);
+            atomicAdd(hash + rec_ev[r], (unsigned long long)t.h);
+            atomicAdd(count + rec_ev[r], t.cnt);
         }
     }
 }
@@ -1152,16 +1158,18 @@ __global__ void k_event_keys(const float *__restrict__ lors, int64_t n_lors, int
     key[j] = k;
 }
 
-// Records permuted into the order `idx`; their event ids (the records' own
-// ev field) are written beside them for the backprojector's value gathers.
-__global__ void k_permute_recs(const SegRec *__restrict__ recs, const int *__restrict__ idx, int64_t n,
-                               SegRec *__restrict__ out, int *__restrict__ out_ev, float2 *__restrict__ out_w)
+// Records permuted into the order `idx`; their event ids (slot_ev, by slot)
+// are written beside them for the backprojector's value gathers.
+__global__ void k_permute_recs(const SegRec *__restrict__ recs, const int *__restrict__ slot_ev,
+                               const int *__restrict__ idx, int64_t n, SegRec *__restrict__ out,
+                               int *__restrict__ out_ev, float2 *__restrict__ out_w)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const SegRec R = recs[idx[i]];
+    const int j = idx[i];
+    const SegRec R = recs[j];
     out[i] = R;
-    out_ev[i] = R.ev;
+    out_ev[i] = slot_ev[j];
     out_w[i] = make_float2(R.Wbp, R.Wfp);
 }
 
@@ -1306,6 +1314,63 @@ using namespace srf;
 // Estimated work of an item: its records times their slab-count class + 1.
 static inline int64_t item_work(const Item &it) { return (int64_t)it.count * (it.cls + 1); }
 
+// Launch orders of a plan's items; the results do not depend on them (every
+// item writes its own output).  Largest first (estimated work) keeps the tail
+// of a launch made of small items: the projector's order.
+static std::vector<int> launch_order_largest_first(const std::vector<Item> &items)
+{
+    std::vector<int> order(items.size());
+    for (size_t k = 0; k < items.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return item_work(items[(size_t)a]) > item_work(items[(size_t)b]);
+    });
+    return order;
+}
+
+// The backprojector's order.  Workgroup i of a launch runs on XCD i % 8, and
+// each of the 8 XCDs has its own L2.  The items, in tile order (z slowest),
+// are cut into 8 bands of about equal estimated work; band x takes the launch
+// positions i % 8 == x, largest item first, so that one XCD works on a few
+// consecutive tile layers, whose segments belong to mostly the same events:
+// their ratio gathers hit that XCD's L2 more often.  A band that runs out
+// gives its positions to the band with the most items left.
+constexpr int N_XCD = 8;
+static std::vector<int> launch_order_by_xcd(const std::vector<Item> &items)
+{
+    int64_t total = 0;
+    for (const Item &it : items) total += item_work(it);
+    std::vector<std::vector<int>> band(N_XCD);
+    int64_t acc = 0;
+    for (size_t k = 0; k < items.size(); ++k) {
+        const int x = total > 0 ? (int)std::min<int64_t>(N_XCD - 1, acc * N_XCD / total) : 0;
+        band[(size_t)x].push_back((int)k);
+        acc += item_work(items[k]);
+    }
+    for (auto &b : band)
+        std::stable_sort(b.begin(), b.end(),
+                         [&](int a, int c) { return item_work(items[(size_t)a]) > item_work(items[(size_t)c]); });
+    std::vector<int> order;
+    order.reserve(items.size());
+    std::vector<size_t> next(N_XCD, 0);
+    while (order.size() < items.size()) {
+        for (int x = 0; x < N_XCD && order.size() < items.size(); ++x) {
+            int from = x;
+            if (next[(size_t)x] == band[(size_t)x].size()) {
+                size_t most = 0;
+                for (int y = 0; y < N_XCD; ++y) {
+                    const size_t left = band[(size_t)y].size() - next[(size_t)y];
+                    if (left > most) {
+                        most = left;
+                        from = y;
+                    }
+                }
+            }
+            order.push_back(band[(size_t)from][next[(size_t)from]++]);
+        }
+    }
+    return order;
+}
+
 // A device buffer with its capacity in elements (plans that are rebuilt keep
 // their buffers and only grow them).
 template <class T>
@@ -1326,7 +1391,8 @@ struct srf_plan {
     DevBuf<int> rec_ev;
     DevBuf<Item> items;
     DevBuf<int> tile_items;
-    DevBuf<int> order;   // launch order of the items: estimated work, largest first
+    DevBuf<int> order;      // projector's launch order of the items: estimated work, largest first
+    DevBuf<int> order_bp;   // backprojector's: 8 bands of tile layers, one per XCD
     // one-shot timing events around the next tiled kernel (srf_plan_time_next)
     hipEvent_t ev_before = nullptr, ev_after = nullptr;
     DevBuf<int> ev_off;
@@ -1341,6 +1407,7 @@ struct srf_plan {
     DevBuf<int> idx, idx2;
     DevBuf<float2> wpair;   // (Wbp, Wfp) per record in the sorted order
     DevBuf<SegRec> recs2;
+    DevBuf<int> slot_ev;    // event id per record slot, before the sort
 };
 
 namespace {
@@ -1449,6 +1516,7 @@ void free_scratch(srf_plan *P, hipStream_t s)
     dfree(P->idx2, s);
     dfree(P->wpair, s);
     dfree(P->recs2, s);
+    dfree(P->slot_ev, s);
 }
 
 // Destroy: the plan may have been used on any stream, so the device is
@@ -1463,6 +1531,7 @@ void plan_free(srf_plan *P)
     dfree(P->items, s);
     dfree(P->tile_items, s);
     dfree(P->order, s);
+    dfree(P->order_bp, s);
     dfree(P->ev_off, s);
     dfree(P->ev_flag, s);
     dfree(P->partial, s);
@@ -1474,7 +1543,8 @@ void plan_free(srf_plan *P)
 size_t plan_bytes(const srf_plan *P)
 {
     return P->recs.cap * sizeof(SegRec) + P->rec_ev.cap * sizeof(int) + P->items.cap * sizeof(Item) +
-           P->tile_items.cap * sizeof(int) + P->order.cap * sizeof(int) + P->ev_off.cap * sizeof(int) +
+           P->tile_items.cap * sizeof(int) + P->order.cap * sizeof(int) +
+           P->order_bp.cap * sizeof(int) + P->ev_off.cap * sizeof(int) +
            P->ev_flag.cap + P->partial.cap * sizeof(float) + P->tile_buf.cap * sizeof(float);
 }
 
@@ -1542,7 +1612,7 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     LaunchGeom G;
     int rc = make_geom(grid, tof, G);
     if (rc) return rc;
-    if (grid.nx > 65535 || grid.ny > 65535) return SRF_ERR_UNSUPPORTED;   // slab index packs in 16 bits
+    if (grid.nx > 32767 || grid.ny > 32767) return SRF_ERR_UNSUPPORTED;   // slab index packs in 15 bits
     const int prescale = (pow2_edge(G.dy) ? 1 : 0) | (pow2_edge(G.dz) ? 2 : 0);
     TileCfg T;
     T.ntx = (grid.nx + TX - 1) / TX;
@@ -1596,7 +1666,7 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     const int64_t nseg = total;
     // work items from the key counts (keys are tile-major, so a tile's items are contiguous)
     std::vector<Item> items;
-    std::vector<int> order;
+    std::vector<int> order, order_bp;
     std::vector<int> tile_items((size_t)T.ntiles + 1, 0);
     unsigned acc = 0;
     for (int k = 0; k < T.nkeys; ++k) {
@@ -1617,21 +1687,19 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
     PB_TRY(ensure(P->items, items.size(), s));
     PB_TRY(ensure(P->tile_items, tile_items.size(), s));
     PB_TRY(ensure(P->order, items.size(), s));
+    PB_TRY(ensure(P->order_bp, items.size(), s));
     PB_TRY(ensure(P->tile_buf, (size_t)std::max(1, nitems) * TVOX, s));
     if (!items.empty())
         PB_HIP(hipMemcpyAsync(P->items.p, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
     PB_HIP(hipMemcpyAsync(P->tile_items.p, tile_items.data(), tile_items.size() * sizeof(int),
                            hipMemcpyHostToDevice, s));
-    // launch order: estimated work (records times their slab-count class + 1),
-    // largest first, so the tail of a launch is made of small items; the
-    // results do not depend on it (every item writes its own output)
-    order.resize(items.size());
-    for (size_t k = 0; k < items.size(); ++k) order[k] = (int)k;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return item_work(items[(size_t)a]) > item_work(items[(size_t)b]);
-    });
-    if (!order.empty())
+    order = launch_order_largest_first(items);
+    order_bp = launch_order_by_xcd(items);
+    if (!order.empty()) {
         PB_HIP(hipMemcpyAsync(P->order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        PB_HIP(hipMemcpyAsync(P->order_bp.p, order_bp.data(), order_bp.size() * sizeof(int),
+                               hipMemcpyHostToDevice, s));
+    }
     if (n > 0) {
         PB_TRY(ensure(P->sk, (size_t)nseg, s));
         PB_TRY(ensure(P->sk2, (size_t)nseg, s));
@@ -1639,12 +1707,15 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
         PB_TRY(ensure(P->idx2, (size_t)nseg, s));
         PB_TRY(ensure(P->recs2, (size_t)nseg, s));
         PB_TRY(ensure(P->wpair, (size_t)nseg, s));
+        PB_TRY(ensure(P->slot_ev, (size_t)nseg, s));
         if (use_tof)
             hipLaunchKernelGGL(k_bin_emit<true>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, prescale, P->ev_off.p, P->recs.p, P->sk.p);
+                               tof_values, (int)tof_stride, G, T, twin, prescale, P->ev_off.p, P->recs.p, P->slot_ev.p,
+                               P->sk.p);
         else
             hipLaunchKernelGGL(k_bin_emit<false>, dim3(nb), dim3(BIN_THREADS), 0, s, lors, n, (int)lor_stride,
-                               tof_values, (int)tof_stride, G, T, twin, prescale, P->ev_off.p, P->recs.p, P->sk.p);
+                               tof_values, (int)tof_stride, G, T, twin, prescale, P->ev_off.p, P->recs.p, P->slot_ev.p,
+                               P->sk.p);
         // Record order: a stable sort of the slot-ordered records by (key,
         // branch, slab count) -- within one, slot order.  The order (and with
         // it the split into work items and each item's fixed-point scale) is a
@@ -1663,8 +1734,8 @@ int plan_build(srf_plan *P, const float *lors, int64_t n, int64_t lor_stride, co
             PB_TRY(ensure(P->sort_tmp, sort_bytes, s));
             PB_HIP(hipcub::DeviceRadixSort::SortPairs(P->sort_tmp.p, sort_bytes, P->sk.p, P->sk2.p, P->idx.p,
                                                        P->idx2.p, (int)nseg, 0, 7 + key_bits, s));
-            hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs.p, P->idx2.p, nseg,
-                               P->recs2.p, P->rec_ev.p, P->wpair.p);
+            hipLaunchKernelGGL(k_permute_recs, dim3(nbs), dim3(256), 0, s, P->recs.p, P->slot_ev.p, P->idx2.p,
+                               nseg, P->recs2.p, P->rec_ev.p, P->wpair.p);
             PB_HIP(hipGetLastError());
             std::swap(P->recs, P->recs2);
         }
@@ -1893,7 +1964,7 @@ static int plan_bp(srf_plan *P, const float *values, const float *y, const float
         return launch_status();
     }
     mark_before(P, s);
-    PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values, y, p,
+    PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order_bp.p, values, y, p,
                                                                    kind, eps, (int)(mode == SRF_BP_ADJOINT), P->G, P->T,
                                                                    P->rec_ev.p, P->tile_buf.p));
     mark_after(P, s);
@@ -1915,7 +1986,7 @@ int srf_plan_digest(srf_plan *P, srf_bp_mode_t mode, uint64_t *hash, int32_t *co
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s, (unsigned *)count, 0u,
                        P->n);
     if (P->nitems > 0)
-        PLAN_DISPATCH(k_digest_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p,
+        PLAN_DISPATCH(k_digest_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->rec_ev.p, P->items.p,
                                                                            (int)(mode == SRF_BP_ADJOINT), P->G,
                                                                            P->T, (unsigned long long *)hash, count));
     return launch_status();
@@ -1940,7 +2011,7 @@ int srf_plan_backproject_update(srf_plan *P, const float *values, float *x, cons
         return launch_status();
     }
     mark_before(P, s);
-    PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order.p, values,
+    PLAN_DISPATCH(k_bp_tiled, <<<P->nitems, TILE_THREADS, 0, s>>>(P->recs.p, P->items.p, P->order_bp.p, values,
                                                                    nullptr, nullptr, 0, 0.0f, 0, P->G, P->T,
                                                                    P->rec_ev.p, P->tile_buf.p));
     mark_after(P, s);
