@@ -169,12 +169,13 @@ int srf_image_from_kernel_layout(const float *src_kernel, float *dst_xyz, int32_
  * float round-off (summation order differs).
  *
  * srf_plan_create, srf_plan_rebuild and srf_plan_destroy are the plan entry
- * points that allocate or free device memory (the binned records, about 64 B
- * per event-tile segment, plus per-item tile buffers); srf_plan_info reports
- * the total.  The LOR array is read during create / rebuild only, which
- * synchronise `stream`.  An event list may bin into at most 2^31 - 1
- * segments (SRF_ERR_UNSUPPORTED beyond: split it into several plans, as OSEM
- * subsets are).  The memory comes from a stream-ordered pool per
+ * points that allocate or free device memory (about 56 B per event-tile
+ * segment: the 48-byte record, its event id and its partial sum; plus per-item
+ * tile buffers); srf_plan_info reports the total.  The LOR array is read
+ * during create / rebuild only, which synchronise `stream`.  An event list may
+ * bin into at most 2^31 - 1 segments, and a plan's grid may have at most 32767
+ * cells along x and along y (SRF_ERR_UNSUPPORTED beyond: split the list into
+ * several plans, as OSEM subsets are).  The memory comes from a stream-ordered pool per
  * device that keeps up to 16 GiB of freed plan memory for the next plans
  * (hipMalloc where a device has no pools; srf_plan_pool_trim releases it);
  * srf_plan_destroy first waits for the device, as hipFree does. */
