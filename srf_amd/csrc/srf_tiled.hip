@@ -520,20 +520,17 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         asm volatile("" ::"v"(i1), "v"(i2a), "v"(i2b), "v"(i3));
         // crossing fractions: reciprocal estimates, exact quotients only where
         // the estimate cannot decide rs > rz (which picks the middle voxel);
-        // rs = 1 (rs0 = 0) along an axis that is not crossed
+        // rs = 1 along an axis that is not crossed
         const float dsv = sh - sl, dzv = zh - zl;
         const float ns_ = (float)cs2 - sl, nz_ = (float)cz2 - zl;
         const float es_ = ns_ * __builtin_amdgcn_rcpf(dsv), ez_ = nz_ * __builtin_amdgcn_rcpf(dzv);
         float rs = bs ? es_ : 1.0f, rz = bz ? ez_ : 1.0f;
-        float rs0 = bs ? es_ : 0.0f, rz0 = bz ? ez_ : 0.0f;
         float dr = rs - rz;
         // rs, rz lie in (0, 1]: an absolute margin is at least as strict as the
         // relative one the estimates' few-ulp error needs
         if (bs && bz && !(fabsf(dr) > 4e-6f)) {
             rs = ns_ / dsv;
             rz = nz_ / dzv;
-            rs0 = rs;
-            rz0 = rz;
             dr = rs - rz;
         }
         asm("" : "+v"(dr));   // keeps |dr| a source modifier of its uses
@@ -541,14 +538,20 @@ __device__ __forceinline__ void march_tile(const SegRec &R, const LaunchGeom &G,
         const int i2 = sgt ? i2a : i2b;
         // the three terms' fractions (the reference's branch tree, flattened):
         // r1 = min(rs, rz), r2 = |rs - rz| (both axes cross), r3 = 1 - m with m
-        // the largest fraction of an axis that crosses
+        // the largest fraction of an axis that crosses: with one axis crossing
+        // that is r1 (the other axis's fraction is 1), with none term 3 is not
+        // applied
         const float r1 = min_nonneg(rs, rz);
         const float r2 = fabsf(dr);
-        const float m = max_nonneg(rs0, rz0);
+        const float m = (bs && bz) ? max_nonneg(rs, rz) : r1;
         float e = 1.0f;
         if (TOF) {
             const float dd = fmaf(pp1, R.tA, R.tB);
             e = __builtin_amdgcn_exp2f(-(dd * dd));
+            // the projector computes the weight before its gathers, so that the
+            // exponential's latency is not exposed at the accumulation after
+            // them
+            if (MODE == M_GATHER) asm volatile("" ::"v"(e));
         }
         if (MODE == M_GATHER) {
             const char *L = reinterpret_cast<const char *>(t.lds);
