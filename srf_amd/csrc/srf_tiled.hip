@@ -1051,7 +1051,7 @@ __global__ __launch_bounds__(256) void k_item_wmax(const float2 *__restrict__ w,
 // The 256 events of a block own one contiguous range of partials: it is
 // staged in LDS with coalesced loads (when it fits), then every thread sums
 // its event's partials in order -- the same additions as a direct loop.
-constexpr int SEG_LOADS = 8;      // staging loads in flight per thread
+constexpr int SEG_LOADS = 12;     // staging loads in flight per thread (one round covers 3072 partials)
 constexpr int SEG_STAGE = 6144;   // partials per block staged (24 KiB; ~2300 at 9 segments per event)
 
 __global__ __launch_bounds__(256) void k_seg_reduce(const float *__restrict__ partial, const int *__restrict__ ev_off,
