@@ -61,6 +61,12 @@ def test_argument_validation_returns_errors(L):
     assert L.srf_em_update(ctypes.c_void_p(18), fake, fake, 4, 0.0, None) == -1   # misaligned
     assert L.srf_lor_digest(fake, 5, 6, None, 0, g, None, 3, fake, fake, None) == -1
     assert L.srf_image_to_kernel_layout(None, fake, 4, 4, 4, None) == -1
+    # plans: a slab index packs in 15 bits, so grids above 32767 cells along x or y are refused
+    plan = ctypes.c_void_p()
+    assert L.srf_plan_create(fake, 5, 6, None, 0, SrfGrid(32768, 8, 8, 0, 0, 0, 32768, 8, 8), None,
+                             ctypes.byref(plan), None) == -2 and plan.value is None
+    assert L.srf_plan_create(fake, 5, 6, None, 0, SrfGrid(8, 40000, 8, 0, 0, 0, 8, 40000, 8), None,
+                             ctypes.byref(plan), None) == -2 and plan.value is None
     # event order: no subsets, short stride, output aliasing the input, too many entries
     assert L.srf_event_order(fake, 5, 6, fake, 5, 0, ctypes.c_void_p(32), None) == -1
     assert L.srf_event_order(fake, 5, 5, fake, 5, 1, ctypes.c_void_p(32), None) == -1
