@@ -94,3 +94,14 @@ def test_graph_replayed_when_capture_works():
     b.advance(graph=True)
     b.advance(graph=False)
     assert replays == [1, 1] and calls == ["eager"] and b.done == 3
+
+
+def test_event_order_refuses_host_tensors():
+    """geometric_subset_order runs on the GPU only (srf_event_order): host
+    tensors are refused before the library is called, and an empty index
+    list comes back as it is."""
+    from srf_amd.functions import geometric_subset_order
+    with pytest.raises(ValueError, match="on one GPU"):
+        geometric_subset_order(torch.zeros(10, 6), torch.arange(10), 2)
+    empty = geometric_subset_order(torch.zeros(0, 6), torch.arange(0), 2)
+    assert empty.numel() == 0 and empty.dtype == torch.int64
