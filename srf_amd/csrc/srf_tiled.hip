@@ -13,7 +13,7 @@
 // Binning (once per event list, reused by every iteration and both directions):
 //   k_bin_count  segments per event and per (tile, length class) key
 //   scan         event offsets (hipcub) and key offsets (host, <= 16384 keys)
-//   k_bin_emit   64-byte segment records, grouped by key: records of one tile
+//   k_bin_emit   48-byte segment records, grouped by key: records of one tile
 //                are contiguous and sorted into slab-count classes, then (radix
 //                sort) by branch and slab count, so a wavefront runs one
 //                branch's marcher over similar lengths.
@@ -1019,7 +1019,7 @@ __global__ __launch_bounds__(256) void k_update_empty(const float *__restrict__ 
 
 // Per item, the largest weight constant of its records in each mode.
 // The records' (Wbp, Wfp) pairs are read from the compact copy the
-// permutation writes (8 B per record instead of a 64 B record line).
+// permutation writes (8 B per record instead of a 48 B record).
 __global__ __launch_bounds__(256) void k_item_wmax(const float2 *__restrict__ w, Item *__restrict__ items)
 {
     __shared__ float sb[256], sf[256];
