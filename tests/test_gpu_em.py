@@ -175,3 +175,26 @@ def test_event_order_sorts_each_subset_stably():
     bad[7] = n_lors
     with pytest.raises(SrfError, match="invalid"):
         geometric_subset_order(Lg, bad, n_sub)
+
+
+def test_event_order_edge_cases():
+    """srf_event_order with one subset (the whole list sorted), with more
+    subsets than entries (every entry its own group: nothing moves), and on
+    6-column (non-TOF) LOR rows, which order as their first 6 columns do."""
+    from srf_amd.functions import geometric_subset_order
+    rng = np.random.default_rng(12)
+    n = 20_000
+    a = rng.uniform(0, 2 * np.pi, (n, 2))
+    z = rng.uniform(-150, 150, (n, 2))
+    L7 = np.stack([300 * np.cos(a[:, 0]), 300 * np.sin(a[:, 0]), z[:, 0],
+                   300 * np.cos(a[:, 1]), 300 * np.sin(a[:, 1]), z[:, 1], rng.normal(0, 50, n)], 1)
+    L7 = torch.from_numpy(L7.astype(np.float32)).cuda()
+    perm = torch.from_numpy(rng.permutation(n)).cuda()
+    one = geometric_subset_order(L7, perm, 1).cpu().numpy()
+    key, near = _order_keys(L7.cpu().numpy()[one], np.zeros(n, np.int64), n, 1)
+    ok = ~near[1:] & ~near[:-1]
+    assert np.array_equal(np.sort(one), np.arange(n)) and bool((key[1:] >= key[:-1])[ok].all())
+    few = perm[:7]
+    assert torch.equal(geometric_subset_order(L7, few, 50), few)
+    six = geometric_subset_order(L7[:, :6].contiguous(), perm, 4)
+    assert torch.equal(six, geometric_subset_order(L7, perm, 4))
